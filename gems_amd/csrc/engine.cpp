@@ -1,0 +1,545 @@
+// Host runtime of the C ABI (include/plk.h): context, HBM buffers, op validation and
+// kernel launches on the context's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine_common.h"
+
+namespace plk {
+
+static thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+    g_last_error = msg;
+    return code;
+}
+void clear_error() { g_last_error.clear(); }
+
+#define HIP_TRY(expr)                                                                           \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return fail(e_ == hipErrorOutOfMemory ? PLK_ERR_NOMEM : PLK_ERR_HIP,                \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                     \
+    } while (0)
+
+static int64_t round_up(int64_t v, int64_t g) { return (v + g - 1) / g * g; }
+
+static bool supported_states(int n) { return n == 4 || n == 20 || n == 61; }
+
+// per-op matrix block: echild for both children + leaf tables for both children
+static int64_t mat_stride_of(int n, int ncat, int su) {
+    return round_up(2LL * ncat * n * n + 2LL * (su + 1) * n * ncat, 32);
+}
+
+static int alloc_stage(Ctx &c, size_t bytes) {
+    if (bytes <= c.h_stage_cap) return PLK_OK;
+    if (c.h_stage) (void)hipHostFree(c.h_stage);
+    c.h_stage = nullptr;
+    c.h_stage_cap = 0;
+    HIP_TRY(hipHostMalloc(&c.h_stage, bytes, hipHostMallocDefault));
+    c.h_stage_cap = bytes;
+    return PLK_OK;
+}
+
+static int ensure_op_capacity(Ctx &c, int nops) {
+    if (nops <= c.ops_cap && nops <= c.mats_cap) return PLK_OK;
+    int cap = std::max(nops, std::max(16, c.ops_cap * 2));
+    if (c.d_ops) (void)hipFree(c.d_ops);
+    if (c.d_jobs) (void)hipFree(c.d_jobs);
+    if (c.d_mats) (void)hipFree(c.d_mats);
+    c.d_ops = nullptr;
+    c.d_jobs = nullptr;
+    c.d_mats = nullptr;
+    c.ops_cap = c.mats_cap = 0;
+    HIP_TRY(hipMalloc(&c.d_ops, sizeof(DevOp) * cap));
+    HIP_TRY(hipMalloc(&c.d_jobs, sizeof(PJob) * 2 * cap));
+    HIP_TRY(hipMalloc(&c.d_mats, c.real_size() * c.mat_stride * cap));
+    c.ops_cap = c.mats_cap = cap;
+    return PLK_OK;
+}
+
+static KParams make_params(Ctx &c) {
+    KParams p{};
+    p.n = c.n;
+    p.ncat = c.ncat;
+    p.block = c.block;
+    p.nptn = c.nptn;
+    p.nptn_pad = c.nptn_pad;
+    p.ntiles = c.ntiles;
+    p.su = c.su;
+    p.safe = c.safe;
+    p.ncw = std::min(c.ncat, 4);
+    p.part = c.d_part;
+    p.slot_stride = (int64_t)c.nptn_pad * c.block;
+    p.scale = c.d_scale;
+    p.scale_stride = (int64_t)c.nptn_pad * (c.safe ? c.ncat : 1);
+    p.states = c.d_states;
+    p.freq = c.d_freq;
+    p.invar = c.d_invar;
+    p.mats = c.d_mats;
+    p.mat_stride = c.mat_stride;
+    p.inv_evec = c.d_inv_evec;
+    p.eval = c.d_eval;
+    p.rates = c.d_rates;
+    p.props = c.d_props;
+    p.tip_lh = c.d_tip;
+    p.evec = c.d_evec;
+    p.ops = c.d_ops;
+    p.nops = 0;
+    p.ptn_lh = c.d_ptn_lh;
+    p.theta = c.d_theta;
+    p.buf_scale = c.d_buf_scale;
+    p.tile_sums = c.d_tile_sums;
+    p.counter = c.d_counter;
+    p.result = c.d_result;
+    return p;
+}
+
+static void timing_begin(Ctx &c, int kind) {
+    if (c.timing) (void)hipEventRecord(c.ev[2 * kind], c.stream);
+}
+static void timing_end(Ctx &c, int kind) {
+    if (c.timing) (void)hipEventRecord(c.ev[2 * kind + 1], c.stream);
+}
+// called after the stream is synchronised
+static void timing_collect(Ctx &c, const bool used[3]) {
+    if (!c.timing) return;
+    for (int k = 0; k < 3; k++) {
+        if (!used[k]) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c.ev[2 * k], c.ev[2 * k + 1]) == hipSuccess) {
+            c.t_total[k] += ms;
+            c.t_count[k] += 1;
+        }
+    }
+}
+
+// Upload the queued ops, run the P(t) kernel, then one traversal launch in `mode`.
+static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
+    const int nops = (int)c.pending.size();
+    int rc = ensure_op_capacity(c, std::max(nops, 1));
+    if (rc) return rc;
+    p.ops = c.d_ops;
+    p.mats = c.d_mats;
+    p.nops = nops;
+    if (nops > 0) {
+        size_t ob = sizeof(DevOp) * nops, jb = sizeof(PJob) * c.jobs.size();
+        rc = alloc_stage(c, ob + jb);
+        if (rc) return rc;
+        std::memcpy(c.h_stage, c.pending.data(), ob);
+        std::memcpy((char *)c.h_stage + ob, c.jobs.data(), jb);
+        HIP_TRY(hipMemcpyAsync(c.d_ops, c.h_stage, ob, hipMemcpyHostToDevice, c.stream));
+        HIP_TRY(hipMemcpyAsync(c.d_jobs, (char *)c.h_stage + ob, jb, hipMemcpyHostToDevice, c.stream));
+        timing_begin(c, 1);
+        HIP_TRY(f64_pmatrix(p, c.d_jobs, (int)c.jobs.size(), c.stream));
+        timing_end(c, 1);
+        used[1] = true;
+    }
+    if (nops > 0 || mode != MODE_PARTIALS) {
+        timing_begin(c, 0);
+        HIP_TRY(f64_traverse(p, mode, c.stream));
+        timing_end(c, 0);
+        used[0] = true;
+    }
+    c.updates += (int64_t)nops * c.nptn;
+    c.pending.clear();
+    c.jobs.clear();
+    return PLK_OK;
+}
+
+static int check_ctx(plk_ctx *ctx) {
+    if (!ctx) return fail(PLK_ERR_ARG, "null context");
+    if (!ctx->model_set || !ctx->patterns_set) return fail(PLK_ERR_STATE, "model and patterns must be set first");
+    return PLK_OK;
+}
+
+static bool valid_ref(const Ctx &c, int r) { return r >= 0 ? r < c.nslots : (-r - 1) < c.ntaxa; }
+
+}  // namespace plk
+
+using namespace plk;
+
+extern "C" const char *plk_last_error(void) { return plk::g_last_error.c_str(); }
+extern "C" int plk_version(void) { return 1; }
+
+extern "C" int plk_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int64_t plk_memory_required(int nstates, int ncat, int ntaxa, int nptn, int nslots, int precision,
+                                       int scaling) {
+    int64_t pad = round_up(nptn, 64), block = (int64_t)nstates * ncat, rs = precision == PLK_FP64 ? 8 : 4;
+    int64_t su = nstates == 4 ? 18 : nstates == 20 ? 23 : nstates;
+    int64_t part = (int64_t)nslots * pad * block * rs;
+    int64_t scale = (int64_t)nslots * pad * (precision == PLK_FP64 ? (scaling ? ncat : 1) * 2 : 4);
+    int64_t theta = pad * block * rs;
+    int64_t states = (int64_t)ntaxa * pad;
+    int64_t mats = (int64_t)(ntaxa) * mat_stride_of(nstates, ncat, (int)su) * rs;
+    int64_t misc = pad * 8 * 4 + (pad / 64 + 1) * 16;
+    return part + scale + theta + states + mats + misc;
+}
+
+extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int nptn, int nslots, int precision,
+                          int scaling) {
+    clear_error();
+    if (!out) return fail(PLK_ERR_ARG, "plk_create: null out");
+    *out = nullptr;
+    if (!supported_states(nstates)) return fail(PLK_ERR_ARG, "plk_create: supported state counts are 4, 20, 61");
+    if (ncat < 1 || ncat > PLK_GPU_MAX_CATS) return fail(PLK_ERR_ARG, "plk_create: ncat must be in [1, 8]");
+    if (ntaxa < 3 || nptn < 1 || nslots < 1) return fail(PLK_ERR_ARG, "plk_create: need ntaxa >= 3, nptn >= 1, nslots >= 1");
+    if (precision != PLK_FP64) return fail(PLK_ERR_ARG, "plk_create: precision form not available");
+    if (scaling != PLK_SCALE_NORM && scaling != PLK_SCALE_SAFE) return fail(PLK_ERR_ARG, "plk_create: bad scaling");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(PLK_ERR_HIP, "plk_create: no HIP device visible (the engine has no CPU path)");
+    auto *c = new plk_ctx();
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    c->dev = dev;
+    c->n = nstates;
+    c->ncat = ncat;
+    c->block = nstates * ncat;
+    c->ntaxa = ntaxa;
+    c->nptn = nptn;
+    c->nptn_pad = (int)round_up(nptn, 64);
+    c->ntiles = c->nptn_pad / 64;
+    c->nslots = nslots;
+    c->prec = precision;
+    c->safe = scaling;
+    c->su = nstates == 4 ? 18 : nstates == 20 ? 23 : nstates;
+    c->mat_stride = mat_stride_of(nstates, ncat, c->su);
+    const size_t rs = c->real_size();
+    const int64_t slot = (int64_t)c->nptn_pad * c->block;
+    const int64_t scale_slot = (int64_t)c->nptn_pad * (scaling ? ncat : 1);
+    auto cleanup = [&](int rc) {
+        plk_destroy(c);
+        return rc;
+    };
+#define CREATE_TRY(expr)                                                                        \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess)                                                                   \
+            return cleanup(fail(e_ == hipErrorOutOfMemory ? PLK_ERR_NOMEM : PLK_ERR_HIP,        \
+                                std::string("plk_create: ") + #expr + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    CREATE_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CREATE_TRY(hipMalloc(&c->d_part, rs * slot * nslots));
+    CREATE_TRY(hipMalloc(&c->d_scale, 2 * scale_slot * nslots));
+    CREATE_TRY(hipMalloc(&c->d_states, (size_t)ntaxa * c->nptn_pad));
+    CREATE_TRY(hipMalloc(&c->d_freq, sizeof(double) * c->nptn_pad));
+    CREATE_TRY(hipMalloc(&c->d_invar, sizeof(double) * c->nptn_pad));
+    CREATE_TRY(hipMalloc(&c->d_theta, rs * slot));
+    CREATE_TRY(hipMalloc(&c->d_buf_scale, sizeof(double) * c->nptn_pad));
+    CREATE_TRY(hipMalloc(&c->d_ptn_lh, sizeof(double) * c->nptn_pad));
+    CREATE_TRY(hipMalloc(&c->d_tile_sums, sizeof(double) * 2 * c->ntiles));
+    CREATE_TRY(hipMalloc(&c->d_result, sizeof(double) * 2));
+    CREATE_TRY(hipMalloc(&c->d_counter, sizeof(unsigned)));
+    const int n = nstates;
+    const size_t model_doubles = n + 2 * n * n + 2 * ncat + (size_t)(c->su + 1) * n;
+    CREATE_TRY(hipMalloc(&c->d_model, sizeof(double) * model_doubles));
+    c->d_eval = c->d_model;
+    c->d_evec = c->d_eval + n;
+    c->d_inv_evec = c->d_evec + n * n;
+    c->d_rates = c->d_inv_evec + n * n;
+    c->d_props = c->d_rates + ncat;
+    c->d_tip = c->d_props + ncat;
+    CREATE_TRY(hipHostMalloc((void **)&c->h_result, sizeof(double) * 2, hipHostMallocDefault));
+    for (int i = 0; i < 6; i++) CREATE_TRY(hipEventCreate(&c->ev[i]));
+    // zero-initialised state: counters, scale counts and partials of unused slots
+    CREATE_TRY(hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), c->stream));
+    CREATE_TRY(hipMemsetAsync(c->d_scale, 0, 2 * scale_slot * nslots, c->stream));
+    CREATE_TRY(hipMemsetAsync(c->d_freq, 0, sizeof(double) * c->nptn_pad, c->stream));
+    CREATE_TRY(hipMemsetAsync(c->d_invar, 0, sizeof(double) * c->nptn_pad, c->stream));
+    CREATE_TRY(hipStreamSynchronize(c->stream));
+    int rc = ensure_op_capacity(*c, std::max(16, ntaxa));
+    if (rc) return cleanup(rc);
+#undef CREATE_TRY
+    *out = c;
+    return PLK_OK;
+}
+
+extern "C" int plk_destroy(plk_ctx *c) {
+    if (!c) return PLK_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->d_part, c->d_scale, c->d_states, c->d_freq, c->d_invar, c->d_mats, c->d_model,
+                    c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
+                    c->d_result, c->d_counter};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    if (c->h_result) (void)hipHostFree(c->h_result);
+    for (int i = 0; i < 6; i++)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PLK_OK;
+}
+
+extern "C" int plk_set_patterns(plk_ctx *c, const uint8_t *states, const double *ptn_freq, const double *ptn_invar) {
+    clear_error();
+    if (!c || !states || !ptn_freq) return fail(PLK_ERR_ARG, "plk_set_patterns: null argument");
+    if (c->su < 0) return fail(PLK_ERR_STATE, "plk_set_patterns: bad context");
+    // padded patterns are STATE_UNKNOWN with zero weight (tree/phylokernelnew.h:1706-1708)
+    std::vector<uint8_t> st((size_t)c->ntaxa * c->nptn_pad, (uint8_t)c->su);
+    for (int t = 0; t < c->ntaxa; t++)
+        for (int p = 0; p < c->nptn; p++) {
+            uint8_t s = states[(size_t)t * c->nptn + p];
+            if (s > c->su) return fail(PLK_ERR_ARG, "plk_set_patterns: state above STATE_UNKNOWN");
+            st[(size_t)t * c->nptn_pad + p] = s;
+        }
+    std::vector<double> f(c->nptn_pad, 0.0), inv(c->nptn_pad, 0.0);
+    std::copy(ptn_freq, ptn_freq + c->nptn, f.begin());
+    if (ptn_invar) std::copy(ptn_invar, ptn_invar + c->nptn, inv.begin());
+    HIP_TRY(hipMemcpyAsync(c->d_states, st.data(), st.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_freq, f.data(), sizeof(double) * f.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_invar, inv.data(), sizeof(double) * inv.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->host_freq = f;
+    c->patterns_set = true;
+    c->theta_valid = false;
+    return PLK_OK;
+}
+
+extern "C" int plk_set_model(plk_ctx *c, const double *eval, const double *evec, const double *inv_evec,
+                             const double *rates, const double *props, int state_unknown, const double *tip_lh) {
+    clear_error();
+    if (!c || !eval || !evec || !inv_evec || !rates || !props || !tip_lh)
+        return fail(PLK_ERR_ARG, "plk_set_model: null argument");
+    if (state_unknown != c->su) return fail(PLK_ERR_ARG, "plk_set_model: state_unknown does not match the state count");
+    const int n = c->n, ncat = c->ncat;
+    std::vector<double> m;
+    m.insert(m.end(), eval, eval + n);
+    m.insert(m.end(), evec, evec + n * n);
+    m.insert(m.end(), inv_evec, inv_evec + n * n);
+    m.insert(m.end(), rates, rates + ncat);
+    m.insert(m.end(), props, props + ncat);
+    m.insert(m.end(), tip_lh, tip_lh + (size_t)(c->su + 1) * n);
+    for (double v : m)
+        if (!std::isfinite(v)) return fail(PLK_ERR_ARG, "plk_set_model: non-finite model value");
+    HIP_TRY(hipMemcpyAsync(c->d_model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->model_set = true;
+    c->theta_valid = false;
+    return PLK_OK;
+}
+
+extern "C" int plk_update_partials(plk_ctx *c, const plk_op *ops, int nops) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nops < 0 || (nops > 0 && !ops)) return fail(PLK_ERR_ARG, "plk_update_partials: bad op list");
+    for (int i = 0; i < nops; i++) {
+        const plk_op &o = ops[i];
+        if (o.out_slot < 0 || o.out_slot >= c->nslots || !valid_ref(*c, o.child[0]) || !valid_ref(*c, o.child[1]) ||
+            o.child[0] == o.out_slot || o.child[1] == o.out_slot || o.reserved != 0 ||
+            !std::isfinite(o.len[0]) || !std::isfinite(o.len[1]) || o.len[0] < 0 || o.len[1] < 0)
+            return fail(PLK_ERR_ARG, "plk_update_partials: invalid op " + std::to_string(i));
+        if (o.child[0] >= 0 && o.child[0] == o.child[1])
+            return fail(PLK_ERR_ARG, "plk_update_partials: both children use one slot (op " + std::to_string(i) + ")");
+        int k = (int)c->pending.size();
+        DevOp d{};
+        d.out_slot = o.out_slot;
+        bool t0 = o.child[0] < 0, t1 = o.child[1] < 0;
+        int side_of0 = 0, side_of1 = 1;
+        if (t0 && t1) {
+            d.kind = KIND_TT;
+            d.left = -o.child[0] - 1;
+            d.right = -o.child[1] - 1;
+        } else if (t0 || t1) {
+            // the tip is always "left" (tree/phylokernelnew.h:1399-1409)
+            d.kind = KIND_TI;
+            d.left = t0 ? -o.child[0] - 1 : -o.child[1] - 1;
+            d.right = t0 ? o.child[1] : o.child[0];
+            if (t1) {
+                side_of0 = 1;
+                side_of1 = 0;
+            }
+        } else {
+            d.kind = KIND_II;
+            d.left = o.child[0];
+            d.right = o.child[1];
+        }
+        c->pending.push_back(d);
+        c->jobs.push_back(PJob{k, side_of0, t0 ? 1 : 0, 0, o.len[0]});
+        c->jobs.push_back(PJob{k, side_of1, t1 ? 1 : 0, 0, o.len[1]});
+    }
+    c->theta_valid = false;
+    return PLK_OK;
+}
+
+extern "C" int plk_flush(plk_ctx *c) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (c->pending.empty()) return PLK_OK;
+    KParams p = make_params(*c);
+    bool used[3] = {false, false, false};
+    rc = run_traversal(*c, p, MODE_PARTIALS, used);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    timing_collect(*c, used);
+    return PLK_OK;
+}
+
+static int branch_refs(Ctx &c, int &a, int &b) {
+    if (!valid_ref(c, a) || !valid_ref(c, b)) return fail(PLK_ERR_ARG, "branch: node reference out of range");
+    if (a < 0 && b < 0) return fail(PLK_ERR_ARG, "branch: both ends are tips");
+    if (b < 0) std::swap(a, b);
+    if (a == b) return fail(PLK_ERR_ARG, "branch: both ends use one slot");
+    return PLK_OK;
+}
+
+// sum of f * fixed pattern values (host fallback of tree/phylokernelnew.h:3167-3184)
+static double fixup_nonfinite(Ctx &c, std::vector<double> &pl) {
+    double s = 0.0;
+    for (int p = 0; p < c.nptn; p++) {
+        if (!std::isfinite(pl[p])) pl[p] = kLogScalingThreshold * 4;
+        s += pl[p] * c.host_freq[p];
+    }
+    return s;
+}
+
+extern "C" int plk_branch_loglik(plk_ctx *c, int a, int b, double t, double *logl, double *ptn_lh) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!logl || !std::isfinite(t) || t < 0) return fail(PLK_ERR_ARG, "plk_branch_loglik: bad arguments");
+    rc = branch_refs(*c, a, b);
+    if (rc) return rc;
+    KParams p = make_params(*c);
+    p.br_a = a;
+    p.br_b = b;
+    p.br_len = t;
+    bool used[3] = {false, false, false};
+    rc = run_traversal(*c, p, MODE_LOGL, used);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    timing_collect(*c, used);
+    double v = c->h_result[0];
+    if (ptn_lh || !std::isfinite(v)) {
+        std::vector<double> pl(c->nptn);
+        HIP_TRY(hipMemcpy(pl.data(), c->d_ptn_lh, sizeof(double) * c->nptn, hipMemcpyDeviceToHost));
+        if (!std::isfinite(v)) v = fixup_nonfinite(*c, pl);
+        if (ptn_lh) std::copy(pl.begin(), pl.end(), ptn_lh);
+    }
+    *logl = v;
+    return PLK_OK;
+}
+
+extern "C" int plk_branch_derivs(plk_ctx *c, int a, int b, double t, int reuse_theta, double *df, double *ddf) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!df || !ddf || !std::isfinite(t) || t < 0) return fail(PLK_ERR_ARG, "plk_branch_derivs: bad arguments");
+    rc = branch_refs(*c, a, b);
+    if (rc) return rc;
+    KParams p = make_params(*c);
+    p.br_a = a;
+    p.br_b = b;
+    p.br_len = t;
+    bool used[3] = {false, false, false};
+    const bool reuse = reuse_theta && c->theta_valid && c->theta_a == a && c->theta_b == b && c->pending.empty();
+    if (reuse) {
+        timing_begin(*c, 2);
+        HIP_TRY(f64_theta_eval(p, 1, c->stream));
+        timing_end(*c, 2);
+        used[2] = true;
+    } else {
+        rc = run_traversal(*c, p, MODE_THETA, used);
+        if (rc) return rc;
+        c->theta_valid = true;
+        c->theta_a = a;
+        c->theta_b = b;
+    }
+    HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    timing_collect(*c, used);
+    double d1 = c->h_result[0], d2 = c->h_result[1];
+    if (!std::isfinite(d1)) {
+        *df = *ddf = 0.0;
+        // NORM: the reference stops with "use -safe" (2594-2596); SAFE warns and zeroes (2646-2649)
+        if (!c->safe) return fail(PLK_ERR_NUMERIC, "Numerical underflow (lh-derivative): use SAFE scaling");
+        return PLK_OK;
+    }
+    *df = d1;
+    *ddf = d2;
+    return PLK_OK;
+}
+
+extern "C" int plk_loglik_from_theta(plk_ctx *c, double t, double *logl) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!logl || !std::isfinite(t) || t < 0) return fail(PLK_ERR_ARG, "plk_loglik_from_theta: bad arguments");
+    if (!c->theta_valid) return fail(PLK_ERR_STATE, "plk_loglik_from_theta: no theta buffer (call plk_branch_derivs)");
+    KParams p = make_params(*c);
+    p.br_len = t;
+    HIP_TRY(f64_theta_eval(p, 0, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double v = c->h_result[0];
+    if (!std::isfinite(v)) {
+        if (!c->safe) return fail(PLK_ERR_NUMERIC, "Numerical underflow (lh-from-buffer): use SAFE scaling");
+        std::vector<double> pl(c->nptn);
+        HIP_TRY(hipMemcpy(pl.data(), c->d_ptn_lh, sizeof(double) * c->nptn, hipMemcpyDeviceToHost));
+        v = fixup_nonfinite(*c, pl);
+    }
+    *logl = v;
+    return PLK_OK;
+}
+
+extern "C" int plk_get_partial(plk_ctx *c, int slot, double *partial, uint16_t *scale) {
+    clear_error();
+    if (!c || slot < 0 || slot >= c->nslots) return fail(PLK_ERR_ARG, "plk_get_partial: bad slot");
+    int rc = plk_flush(c);
+    if (rc) return rc;
+    const int64_t slot_len = (int64_t)c->nptn_pad * c->block;
+    if (partial) {
+        std::vector<double> raw(slot_len);
+        HIP_TRY(hipMemcpy(raw.data(), (char *)c->d_part + 8 * slot_len * slot, 8 * slot_len, hipMemcpyDeviceToHost));
+        for (int p = 0; p < c->nptn; p++)
+            for (int i = 0; i < c->block; i++)
+                partial[(int64_t)p * c->block + i] = raw[((int64_t)(p / 64) * c->block + i) * 64 + p % 64];
+    }
+    if (scale) {
+        const int64_t ss = (int64_t)c->nptn_pad * (c->safe ? c->ncat : 1);
+        std::vector<uint16_t> raw(ss);
+        HIP_TRY(hipMemcpy(raw.data(), (char *)c->d_scale + 2 * ss * slot, 2 * ss, hipMemcpyDeviceToHost));
+        std::copy(raw.begin(), raw.begin() + (int64_t)c->nptn * (c->safe ? c->ncat : 1), scale);
+    }
+    return PLK_OK;
+}
+
+extern "C" int plk_set_timing(plk_ctx *c, int enable) {
+    if (!c) return fail(PLK_ERR_ARG, "null context");
+    c->timing = enable != 0;
+    for (int k = 0; k < 3; k++) {
+        c->t_total[k] = 0;
+        c->t_count[k] = 0;
+    }
+    return PLK_OK;
+}
+
+extern "C" int plk_get_timing(plk_ctx *c, int kind, double *total_ms, int64_t *launches) {
+    if (!c || kind < 0 || kind > 2) return fail(PLK_ERR_ARG, "plk_get_timing: bad arguments");
+    if (total_ms) *total_ms = c->t_total[kind];
+    if (launches) *launches = c->t_count[kind];
+    return PLK_OK;
+}
+
+extern "C" int64_t plk_updates(plk_ctx *c) { return c ? c->updates : 0; }
+
+extern "C" int plk_synchronize(plk_ctx *c) {
+    if (!c) return fail(PLK_ERR_ARG, "null context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return PLK_OK;
+}

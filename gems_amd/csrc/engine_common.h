@@ -1,0 +1,111 @@
+// Shared host/device declarations of the likelihood engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "plk.h"
+#include "plk_internal.h"
+
+// Largest category count the GPU kernels hold per pattern in LDS.
+#define PLK_GPU_MAX_CATS 8
+
+namespace plk {
+
+// Device-side op: one directional partial update.  kind: 0 tip-tip, 1 tip-internal
+// (left is the tip), 2 internal-internal (tree/phylokernelnew.h:1399-1409, 1636, 1752, 1911).
+struct DevOp {
+    int32_t out_slot;
+    int32_t kind;
+    int32_t left;   // taxon (kind 0/1) or slot (kind 2)
+    int32_t right;  // taxon (kind 0) or slot (kind 1/2)
+};
+
+// One transition-matrix job of the P(t) kernel: op index, child side, leaf flag, length.
+struct PJob {
+    int32_t op;
+    int32_t side;
+    int32_t leaf;
+    int32_t pad;
+    double len;
+};
+
+enum : int { KIND_TT = 0, KIND_TI = 1, KIND_II = 2 };
+enum : int { MODE_PARTIALS = 0, MODE_LOGL = 1, MODE_THETA = 2 };
+
+// Parameters of every likelihood kernel (passed by value).
+struct KParams {
+    int n, ncat, block, nptn, nptn_pad, ntiles, su, safe, ncw;  // ncw = waves per tile
+    void *part;               // slots x slot_stride reals
+    int64_t slot_stride;      // reals per slot
+    void *scale;              // slots x scale_stride counters (uint16 FP64 / int32 FP32)
+    int64_t scale_stride;
+    const uint8_t *states;    // ntaxa x nptn_pad
+    const double *freq, *invar;
+    const void *mats;         // per-op matrices
+    int64_t mat_stride;       // reals per op
+    const double *inv_evec, *eval, *rates, *props, *tip_lh, *evec;
+    const DevOp *ops;
+    int nops;
+    int br_a, br_b;           // branch stage: node refs (br_a may be a tip), br_b a slot
+    double br_len;
+    double *ptn_lh;           // nptn_pad
+    void *theta;              // nptn_pad x block reals
+    double *buf_scale;        // nptn_pad
+    double *tile_sums;        // ntiles x 2
+    unsigned *counter;
+    double *result;           // 2
+};
+
+// Host-side context (opaque plk_ctx).
+struct Ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    int n = 0, ncat = 0, block = 0, ntaxa = 0, nptn = 0, nptn_pad = 0, ntiles = 0, nslots = 0;
+    int prec = PLK_FP64, safe = 0, su = -1;
+    bool model_set = false, patterns_set = false;
+    void *d_part = nullptr;
+    void *d_scale = nullptr;
+    uint8_t *d_states = nullptr;
+    double *d_freq = nullptr, *d_invar = nullptr;
+    void *d_mats = nullptr;
+    int64_t mat_stride = 0;
+    int mats_cap = 0;
+    double *d_model = nullptr;  // eval | evec | inv_evec | rates | props | tip_lh
+    double *d_eval = nullptr, *d_evec = nullptr, *d_inv_evec = nullptr, *d_rates = nullptr, *d_props = nullptr,
+           *d_tip = nullptr;
+    DevOp *d_ops = nullptr;
+    PJob *d_jobs = nullptr;
+    int ops_cap = 0;
+    void *d_theta = nullptr;
+    double *d_buf_scale = nullptr, *d_ptn_lh = nullptr, *d_tile_sums = nullptr, *d_result = nullptr;
+    unsigned *d_counter = nullptr;
+    // host staging (pinned)
+    void *h_stage = nullptr;
+    size_t h_stage_cap = 0;
+    double *h_result = nullptr;
+    std::vector<DevOp> pending;
+    std::vector<PJob> jobs;
+    int theta_a = 0, theta_b = 0;
+    bool theta_valid = false;
+    bool timing = false;
+    hipEvent_t ev[6] = {};
+    double t_total[3] = {0, 0, 0};
+    int64_t t_count[3] = {0, 0, 0};
+    int64_t updates = 0;
+    std::vector<double> host_freq;
+    size_t real_size() const { return prec == PLK_FP64 ? 8 : 4; }
+};
+
+}  // namespace plk
+
+struct plk_ctx : public plk::Ctx {};
+
+namespace plk {
+// Launchers implemented in the kernel translation units.
+hipError_t f64_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
+hipError_t f64_traverse(const KParams &p, int mode, hipStream_t s);
+hipError_t f64_theta_eval(const KParams &p, int deriv, hipStream_t s);
+}  // namespace plk

@@ -1,0 +1,440 @@
+// PhyloTreeGPU: the host half of IQ-TREE's PhyloTree likelihood path, driving the
+// GPU engine through the plk.h C ABI.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "plk.h"
+#include "plk_internal.h"
+#include "plk_tree.h"
+
+namespace plk {
+
+class PhyloTreeGPU {
+  public:
+    plk_ctx *ctx = nullptr;
+    int ntaxa = 0, nnodes = 0;
+    std::vector<int> nei;       // nnodes x 3
+    std::vector<double> len;    // nnodes x 3
+    // LM_PER_NODE: internal node v owns slot v - ntaxa, oriented toward orient[v]
+    std::vector<int> orient;
+    std::vector<char> computed;
+    std::vector<plk_op> ops;
+    int64_t n_derivs = 0, n_ops = 0;
+
+    bool is_leaf(int v) const { return v < ntaxa; }
+    int slot(int v) const { return v - ntaxa; }
+    int ref(int v) const { return is_leaf(v) ? -(v + 1) : slot(v); }
+
+    int dir(int a, int b) const {
+        for (int k = 0; k < 3; k++)
+            if (nei[a * 3 + k] == b) return a * 3 + k;
+        return -1;
+    }
+    double length(int a, int b) const { return len[dir(a, b)]; }
+
+    int subtree_size(int node, int dad) const {
+        if (is_leaf(node)) return 1;
+        int s = 0;
+        for (int k = 0; k < 3; k++) {
+            int x = nei[node * 3 + k];
+            if (x >= 0 && x != dad) s += subtree_size(x, node);
+        }
+        return s;
+    }
+
+    // computeTraversalInfo(PhyloNeighbor *dad_branch, PhyloNode *dad)
+    // (tree/phylotree.cpp:6031-6139): post-order over dirty directions, children in
+    // decreasing subtree size; reorientPartialLh (6011-6025) moves the node's slot.
+    void collect(int dad, int node) {
+        if (is_leaf(node)) return;
+        if (orient[node] == dad && computed[node]) return;
+        orient[node] = dad;
+        computed[node] = 0;
+        int kids[2], nk = 0;
+        for (int k = 0; k < 3; k++) {
+            int x = nei[node * 3 + k];
+            if (x >= 0 && x != dad) kids[nk++] = x;
+        }
+        if (subtree_size(kids[0], node) >= subtree_size(kids[1], node)) {
+            collect(node, kids[0]);
+            collect(node, kids[1]);
+        } else {
+            collect(node, kids[1]);
+            collect(node, kids[0]);
+        }
+        computed[node] = 1;
+        plk_op op{};
+        op.out_slot = slot(node);
+        op.child[0] = ref(kids[0]);
+        op.child[1] = ref(kids[1]);
+        op.len[0] = length(node, kids[0]);
+        op.len[1] = length(node, kids[1]);
+        ops.push_back(op);
+    }
+
+    // both directions of branch (dad, node), then queue the ops
+    int plan(int dad, int node) {
+        ops.clear();
+        collect(node, dad);
+        collect(dad, node);
+        n_ops += (int64_t)ops.size();
+        if (ops.empty()) return PLK_OK;
+        return plk_update_partials(ctx, ops.data(), (int)ops.size());
+    }
+
+    // PhyloNode::clearReversePartialLh (tree/phylonode.cpp:22-33)
+    void clear_reverse(int node, int dad) {
+        for (int k = 0; k < 3; k++) {
+            int x = nei[node * 3 + k];
+            if (x < 0 || x == dad) continue;
+            if (!is_leaf(node) && orient[node] == x) computed[node] = 0;
+            clear_reverse(x, node);
+        }
+    }
+
+    void clear_all() { std::fill(computed.begin(), computed.end(), 0); }
+
+    int set_length(int a, int b, double v) {
+        int da = dir(a, b), db = dir(b, a);
+        if (da < 0 || db < 0) return fail(PLK_ERR_ARG, "no such branch");
+        len[da] = len[db] = v;
+        clear_reverse(a, b);
+        clear_reverse(b, a);
+        return PLK_OK;
+    }
+
+    int loglik(int dad, int node, double *logl, double *ptn_lh) {
+        if (dir(dad, node) < 0) return fail(PLK_ERR_ARG, "no such branch");
+        int rc = plan(dad, node);
+        if (rc) return rc;
+        return plk_branch_loglik(ctx, ref(dad), ref(node), length(dad, node), logl, ptn_lh);
+    }
+
+    int derivs(int dad, int node, int reuse, double *df, double *ddf) {
+        if (dir(dad, node) < 0) return fail(PLK_ERR_ARG, "no such branch");
+        int rc = plan(dad, node);
+        if (rc) return rc;
+        n_derivs++;
+        return plk_branch_derivs(ctx, ref(dad), ref(node), length(dad, node), reuse, df, ddf);
+    }
+
+    // computeFuncDerv (tree/phylotree.cpp:2555-2562): derivatives of -logL
+    int func_derv(int a, int b, double value, bool &first, double &f, double &df) {
+        int da = dir(a, b), db = dir(b, a);
+        len[da] = len[db] = value;
+        double d1 = 0, d2 = 0;
+        int rc = derivs(a, b, first ? 0 : 1, &d1, &d2);
+        first = false;
+        f = -d1;
+        df = -d2;
+        return rc;
+    }
+
+    // Optimization::minimizeNewton (utils/optimization.cpp:422-499)
+    int minimize_newton(int a, int b, double x1, double xguess, double x2, double xacc, int max_nr, double &out) {
+        double df, dx, f, temp, xh, xl, rts, rts_old;
+        bool first = true;
+        rts = std::min(std::max(xguess, x1), x2);
+        int rc = func_derv(a, b, rts, first, f, df);
+        if (rc) return rc;
+        if (!std::isfinite(f) || !std::isfinite(df)) return fail(PLK_ERR_NUMERIC, "Wrong computeFuncDerv");
+        if (df >= 0.0 && std::fabs(f) < xacc) {
+            out = rts;
+            return PLK_OK;
+        }
+        if (f < 0.0) {
+            xl = rts;
+            xh = x2;
+        } else {
+            xh = rts;
+            xl = x1;
+        }
+        dx = std::fabs(xh - xl);
+        for (int j = 1; j <= max_nr; j++) {
+            rts_old = rts;
+            if ((df <= 0.0) || (((rts - xh) * df - f) * ((rts - xl) * df - f) >= 0.0)) {
+                dx = 0.5 * (xh - xl);
+                rts = xl + dx;
+                if (xl == rts) {
+                    out = rts;
+                    return PLK_OK;
+                }
+            } else {
+                dx = f / df;
+                temp = rts;
+                rts -= dx;
+                if (temp == rts) {
+                    out = rts;
+                    return PLK_OK;
+                }
+            }
+            if (std::fabs(dx) < xacc || j == max_nr) {
+                out = rts_old;
+                return PLK_OK;
+            }
+            rc = func_derv(a, b, rts, first, f, df);
+            if (rc) return rc;
+            if (!std::isfinite(f) || !std::isfinite(df)) return fail(PLK_ERR_NUMERIC, "Wrong computeFuncDerv");
+            if (df > 0.0 && std::fabs(f) < xacc) {
+                out = rts;
+                return PLK_OK;
+            }
+            if (f < 0.0) xl = rts;
+            else if (f > 0.0) xh = rts;
+        }
+        return fail(PLK_ERR_NUMERIC, "Maximum number of iterations exceeded in minimizeNewton");
+    }
+
+    // PhyloTree::optimizeOneBranch (tree/phylotree.cpp:2620-2670), Newton branch
+    int optimize_one(int n1, int n2, double min_bl, double max_bl, int max_nr, double &optx) {
+        if (dir(n1, n2) < 0) return fail(PLK_ERR_ARG, "no such branch");
+        double current = length(n1, n2);
+        int rc = minimize_newton(n1, n2, min_bl, current, max_bl, min_bl, max_nr, optx);
+        if (rc) return rc;
+        int da = dir(n1, n2), db = dir(n2, n1);
+        if (optx > max_bl * 0.95) {
+            double opt_lh = 0, orig_lh = 0;
+            rc = plk_loglik_from_theta(ctx, optx, &opt_lh);
+            if (rc) return rc;
+            rc = plk_loglik_from_theta(ctx, current, &orig_lh);
+            if (rc) return rc;
+            if (orig_lh > opt_lh) optx = current;
+        }
+        len[da] = len[db] = optx;
+        if (current != optx) {
+            clear_reverse(n1, n2);
+            clear_reverse(n2, n1);
+        }
+        return PLK_OK;
+    }
+
+    // MTree::findFarthestLeaf (tree/mtree.cpp:3040-3060)
+    int farthest_leaf(int node, int dad, std::vector<int> &h) const {
+        if (dad >= 0 && is_leaf(node)) {
+            h[node] = 0;
+            return node;
+        }
+        int res = -1;
+        h[node] = 0;
+        for (int k = 0; k < 3; k++) {
+            int x = nei[node * 3 + k];
+            if (x < 0 || x == dad) continue;
+            int leaf = farthest_leaf(x, node, h);
+            if (h[node] < h[x] + 1) {
+                h[node] = h[x] + 1;
+                res = leaf;
+            }
+        }
+        return res;
+    }
+
+    // MTree::getPreOrderBranches (tree/mtree.cpp:3062-3082)
+    void preorder(int node, int dad, const std::vector<int> &h, std::vector<int> &a, std::vector<int> &b) const {
+        if (dad >= 0) {
+            a.push_back(node);
+            b.push_back(dad);
+        }
+        int nv[3], m = 0;
+        for (int k = 0; k < 3; k++)
+            if (nei[node * 3 + k] >= 0) nv[m++] = nei[node * 3 + k];
+        for (int i = 0; i < m; i++)
+            for (int j = i + 1; j < m; j++)
+                if (h[nv[i]] > h[nv[j]]) std::swap(nv[i], nv[j]);
+        for (int i = 0; i < m; i++)
+            if (nv[i] != dad) preorder(nv[i], node, h, a, b);
+    }
+
+    // PhyloTree::optimizeAllBranches(iterations, tol, maxNRStep) (tree/phylotree.cpp:2724-2801)
+    int optimize_all(int iterations, double tol, double min_bl, double max_bl, int max_nr, double &out) {
+        std::vector<int> h(nnodes, 0), na, nb;
+        int far = farthest_leaf(0, -1, h);
+        farthest_leaf(far, -1, h);
+        preorder(far, -1, h, na, nb);
+        double tree_lh = 0.0;
+        int rc = loglik(na[0], nb[0], &tree_lh, nullptr);
+        if (rc) return rc;
+        std::vector<double> saved;
+        for (int it = 0; it < iterations; it++) {
+            saved = len;
+            for (size_t j = 0; j < na.size(); j++) {
+                double optx;
+                rc = optimize_one(na[j], nb[j], min_bl, max_bl, max_nr, optx);
+                if (rc) return rc;
+            }
+            double new_lh = 0.0;
+            rc = plk_loglik_from_theta(ctx, length(na.back(), nb.back()), &new_lh);
+            if (rc) return rc;
+            if (new_lh < tree_lh - tol * 0.1) {
+                len = saved;
+                clear_all();
+                rc = loglik(na[0], nb[0], &tree_lh, nullptr);
+                if (rc) return rc;
+                break;
+            }
+            if (tree_lh <= new_lh && new_lh <= tree_lh + tol) {
+                tree_lh = new_lh;
+                break;
+            }
+            tree_lh = new_lh;
+        }
+        out = tree_lh;
+        return PLK_OK;
+    }
+};
+
+}  // namespace plk
+
+struct plk_tree : public plk::PhyloTreeGPU {};
+
+using plk::fail;
+
+extern "C" int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling) {
+    if (!out) return fail(PLK_ERR_ARG, "plk_tree_create: null out");
+    *out = nullptr;
+    if (ntaxa < 3) return fail(PLK_ERR_ARG, "plk_tree_create: need at least 3 taxa");
+    auto *t = new plk_tree();
+    t->ntaxa = ntaxa;
+    t->nnodes = 2 * ntaxa - 2;
+    t->nei.assign((size_t)t->nnodes * 3, -1);
+    t->len.assign((size_t)t->nnodes * 3, 0.0);
+    t->orient.assign(t->nnodes, -1);
+    t->computed.assign(t->nnodes, 0);
+    int rc = plk_create(&t->ctx, nstates, ncat, ntaxa, nptn, ntaxa - 2, precision, scaling);
+    if (rc) {
+        delete t;
+        return rc;
+    }
+    *out = t;
+    return PLK_OK;
+}
+
+extern "C" int plk_tree_destroy(plk_tree *t) {
+    if (!t) return PLK_OK;
+    plk_destroy(t->ctx);
+    delete t;
+    return PLK_OK;
+}
+
+extern "C" plk_ctx *plk_tree_context(plk_tree *t) { return t ? t->ctx : nullptr; }
+
+extern "C" int plk_tree_set_model(plk_tree *t, const double *eval, const double *evec, const double *inv_evec,
+                                  const double *rates, const double *props, int su, const double *tip_lh) {
+    if (!t) return fail(PLK_ERR_ARG, "null tree");
+    t->clear_all();
+    return plk_set_model(t->ctx, eval, evec, inv_evec, rates, props, su, tip_lh);
+}
+
+extern "C" int plk_tree_set_patterns(plk_tree *t, const uint8_t *states, const double *freq, const double *invar) {
+    if (!t) return fail(PLK_ERR_ARG, "null tree");
+    t->clear_all();
+    return plk_set_patterns(t->ctx, states, freq, invar);
+}
+
+extern "C" int plk_tree_set_topology(plk_tree *t, const int32_t *nei, const double *len) {
+    if (!t || !nei || !len) return fail(PLK_ERR_ARG, "plk_tree_set_topology: null argument");
+    // validate: symmetric adjacency, leaves degree 1 (slot 0), internal degree 3
+    for (int v = 0; v < t->nnodes; v++) {
+        int deg = 0;
+        for (int k = 0; k < 3; k++) {
+            int x = nei[v * 3 + k];
+            if (x < -1 || x >= t->nnodes || x == v) return fail(PLK_ERR_ARG, "plk_tree_set_topology: bad neighbour");
+            if (x < 0) continue;
+            deg++;
+            bool back = false;
+            for (int q = 0; q < 3; q++)
+                if (nei[x * 3 + q] == v) back = std::fabs(len[x * 3 + q] - len[v * 3 + k]) == 0.0;
+            if (!back) return fail(PLK_ERR_ARG, "plk_tree_set_topology: adjacency/lengths not symmetric");
+            if (!(len[v * 3 + k] >= 0.0) || !std::isfinite(len[v * 3 + k]))
+                return fail(PLK_ERR_ARG, "plk_tree_set_topology: bad branch length");
+        }
+        if (v < t->ntaxa ? (deg != 1 || nei[v * 3] < 0) : deg != 3)
+            return fail(PLK_ERR_ARG, "plk_tree_set_topology: tree must be binary with leaves 0..ntaxa-1");
+    }
+    t->nei.assign(nei, nei + (size_t)t->nnodes * 3);
+    t->len.assign(len, len + (size_t)t->nnodes * 3);
+    std::fill(t->orient.begin(), t->orient.end(), -1);
+    t->clear_all();
+    return PLK_OK;
+}
+
+extern "C" int plk_tree_set_length(plk_tree *t, int a, int b, double len) {
+    if (!t || a < 0 || b < 0 || a >= t->nnodes || b >= t->nnodes || !(len >= 0.0))
+        return fail(PLK_ERR_ARG, "plk_tree_set_length: bad arguments");
+    return t->set_length(a, b, len);
+}
+
+extern "C" int plk_tree_get_length(plk_tree *t, int a, int b, double *len) {
+    if (!t || !len || a < 0 || b < 0 || a >= t->nnodes || b >= t->nnodes || t->dir(a, b) < 0)
+        return fail(PLK_ERR_ARG, "plk_tree_get_length: bad arguments");
+    *len = t->length(a, b);
+    return PLK_OK;
+}
+
+extern "C" int plk_tree_clear_all(plk_tree *t) {
+    if (!t) return fail(PLK_ERR_ARG, "null tree");
+    t->clear_all();
+    return PLK_OK;
+}
+
+static bool node_ok(plk_tree *t, int a, int b) {
+    return t && a >= 0 && b >= 0 && a < t->nnodes && b < t->nnodes && t->dir(a, b) >= 0;
+}
+
+extern "C" int plk_tree_loglik(plk_tree *t, int dad, int node, double *logl, double *ptn_lh) {
+    if (!node_ok(t, dad, node) || !logl) return fail(PLK_ERR_ARG, "plk_tree_loglik: bad arguments");
+    return t->loglik(dad, node, logl, ptn_lh);
+}
+
+extern "C" int plk_tree_compute_likelihood(plk_tree *t, double *logl) {
+    if (!t || !logl) return fail(PLK_ERR_ARG, "plk_tree_compute_likelihood: bad arguments");
+    // root is leaf 0; current_it = its branch (tree/phylotree.cpp:1258-1299)
+    return t->loglik(0, t->nei[0], logl, nullptr);
+}
+
+extern "C" int plk_tree_derivs(plk_tree *t, int dad, int node, int reuse, double *df, double *ddf) {
+    if (!node_ok(t, dad, node) || !df || !ddf) return fail(PLK_ERR_ARG, "plk_tree_derivs: bad arguments");
+    return t->derivs(dad, node, reuse, df, ddf);
+}
+
+extern "C" int plk_tree_loglik_from_theta(plk_tree *t, int dad, int node, double *logl) {
+    if (!node_ok(t, dad, node) || !logl) return fail(PLK_ERR_ARG, "plk_tree_loglik_from_theta: bad arguments");
+    return plk_loglik_from_theta(t->ctx, t->length(dad, node), logl);
+}
+
+extern "C" int plk_tree_get_partial(plk_tree *t, int dad, int node, double *partial, uint16_t *scale) {
+    if (!node_ok(t, dad, node) || node < t->ntaxa) return fail(PLK_ERR_ARG, "plk_tree_get_partial: bad arguments");
+    t->ops.clear();
+    t->collect(dad, node);
+    if (!t->ops.empty()) {
+        int rc = plk_update_partials(t->ctx, t->ops.data(), (int)t->ops.size());
+        if (rc) return rc;
+    }
+    return plk_get_partial(t->ctx, t->slot(node), partial, scale);
+}
+
+extern "C" int plk_tree_optimize_one_branch(plk_tree *t, int n1, int n2, double min_bl, double max_bl, int max_nr,
+                                            double *optx) {
+    if (!node_ok(t, n1, n2) || !optx) return fail(PLK_ERR_ARG, "plk_tree_optimize_one_branch: bad arguments");
+    return t->optimize_one(n1, n2, min_bl, max_bl, max_nr, *optx);
+}
+
+extern "C" int plk_tree_optimize_all_branches(plk_tree *t, int iterations, double tol, double min_bl, double max_bl,
+                                              int max_nr, double *logl) {
+    if (!t || !logl) return fail(PLK_ERR_ARG, "plk_tree_optimize_all_branches: bad arguments");
+    return t->optimize_all(iterations, tol, min_bl, max_bl, max_nr, *logl);
+}
+
+extern "C" int plk_tree_full_traversal(plk_tree *t, double *logl) {
+    if (!t || !logl) return fail(PLK_ERR_ARG, "plk_tree_full_traversal: bad arguments");
+    t->clear_all();
+    return t->loglik(0, t->nei[0], logl, nullptr);
+}
+
+extern "C" int plk_tree_counters(plk_tree *t, int64_t *out3) {
+    if (!t || !out3) return fail(PLK_ERR_ARG, "plk_tree_counters: bad arguments");
+    out3[0] = plk_updates(t->ctx);
+    out3[1] = t->n_derivs;
+    out3[2] = t->n_ops;
+    return PLK_OK;
+}

@@ -1,0 +1,138 @@
+/*
+ * plk.h -- C ABI of the MI355X Felsenstein likelihood engine (libplk.so).
+ *
+ * The engine replaces the four PhyloTree kernel function pointers that IQ-TREE's
+ * setLikelihoodKernel installs (tree/phylotreesse.cpp:91-199) and calls only through
+ * the wrappers at tree/phylotreesse.cpp:212-236:
+ *
+ *   computePartialLikelihoodPointer     tree/phylotree.h:902-904   -> plk_update_partials
+ *   computeLikelihoodBranchPointer      tree/phylotree.h:948-951   -> plk_branch_loglik
+ *   computeLikelihoodDervPointer        tree/phylotree.h:1343-1346 -> plk_branch_derivs
+ *   computeLikelihoodFromBufferPointer  tree/phylotree.h:1002-1004 -> plk_loglik_from_theta
+ *
+ * plus the buffers PhyloTree owns for them (initializeAllPartialLh,
+ * tree/phylotree.cpp:905-960 and 1085-1213): one context holds the partial slots,
+ * scale counters (scale_num), theta buffer, pattern weights and model arrays of
+ * one alignment on one GPU, resident in HBM.
+ *
+ * Conventions.
+ *   - Every function returns PLK_OK (0) or a negative error code; plk_last_error()
+ *     returns a thread-local message for the last failure.  No function aborts.
+ *   - A "node reference" is a slot id (>= 0) holding a directional partial, or a tip,
+ *     encoded as -(taxon+1).
+ *   - Partials use IQ-TREE's eigen-space representation in the FP64 form
+ *     (tree/phylokernelnew.h:1989-1994) and are laid out on the device exactly like
+ *     the reference's SIMD layout with 64 lanes: element (ptn, c, x) at
+ *     (floor(ptn/64)*block + c*n + x)*64 + ptn%64, block = ncat*n.
+ *   - scale counts follow scale_num (tree/phylonode.h:17, uint16): one per pattern in
+ *     NORM mode, one per (pattern, category) in SAFE mode
+ *     (tree/phylokernelnew.h:1924-1936, mode rule tree/phylotreesse.cpp:95-96).
+ *   - Contexts are not re-entrant (like PhyloTree); different contexts may be used
+ *     from different host threads concurrently, each on its own HIP stream.
+ */
+#ifndef PLK_H
+#define PLK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PLK_OK 0
+#define PLK_ERR_ARG (-1)      /* invalid argument */
+#define PLK_ERR_HIP (-2)      /* HIP runtime failure */
+#define PLK_ERR_STATE (-3)    /* call out of order (e.g. no model / no theta) */
+#define PLK_ERR_NUMERIC (-4)  /* non-finite derivative in NORM mode: the reference's
+                                 "use -safe" outError (tree/phylokernelnew.h:2594-2596) */
+#define PLK_ERR_NOMEM (-5)    /* device memory exhausted */
+
+/* precision form */
+#define PLK_FP64 0            /* reference semantics, eigen-space partials */
+#define PLK_FP32 1            /* FP32 partials, per-pattern power-of-two exponents,
+                                 FP64 eigen system / exponent sums / reductions */
+/* underflow scaling mode (tree/phylotree.h:61-62 SAFE_LH / NORM_LH) */
+#define PLK_SCALE_NORM 0
+#define PLK_SCALE_SAFE 1
+
+typedef struct plk_ctx plk_ctx;
+
+/* One directional partial update: out_slot <- children.  One element of IQ-TREE's
+ * traversal_info (TraversalInfo, tree/phylotree.h:320-331), planned on the host by
+ * computeTraversalInfo (tree/phylotree.cpp:6031-6139). */
+typedef struct plk_op {
+    int32_t out_slot;
+    int32_t child[2];  /* slot id or -(taxon+1) */
+    int32_t reserved;  /* must be 0 */
+    double len[2];     /* branch length from the output node to each child */
+} plk_op;
+
+const char *plk_last_error(void);
+int plk_version(void);
+/* Number of visible GPUs (0 when none); never fails. */
+int plk_device_count(void);
+
+/* PhyloTree::initializeAllPartialLh (tree/phylotree.cpp:905-960).  nslots directional
+ * partial buffers of nptn x ncat x nstates reals each. */
+int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int nptn, int nslots,
+               int precision, int scaling);
+int plk_destroy(plk_ctx *ctx);
+/* Bytes of device memory a context of this shape holds (PhyloTree::getMemoryRequired,
+ * tree/phylotree.cpp:992-1052). */
+int64_t plk_memory_required(int nstates, int ncat, int ntaxa, int nptn, int nslots,
+                            int precision, int scaling);
+
+/* Pattern-compressed alignment: states[taxon*nptn + ptn] (uint8, STATE_UNKNOWN
+ * alignment/alignment.cpp:1116-1128), ptn_freq and ptn_invar
+ * (tree/phylotreesse.cpp:537-612).  invar may be NULL (all zero). */
+int plk_set_patterns(plk_ctx *ctx, const uint8_t *states, const double *ptn_freq,
+                     const double *ptn_invar);
+
+/* Model arrays (ModelSubst::getEigenvalues/getEigenvectors/getInverseEigenvectors,
+ * model/modelsubst.h:363-375; RateHeterogeneity::getRate/getProp model/rategamma.h:106,120)
+ * and the tip table tip_partial_lh[(state_unknown+1) x nstates]
+ * (tree/phylotreesse.cpp:354-375).  All FP64, row-major. */
+int plk_set_model(plk_ctx *ctx, const double *eval, const double *evec, const double *inv_evec,
+                  const double *rates, const double *props, int state_unknown,
+                  const double *tip_lh);
+
+/* computePartialLikelihood over one traversal (tree/phylokernelnew.h:1293-2043): ops in
+ * post-order.  The ops are queued and run fused with the next branch call, or at
+ * plk_flush. */
+int plk_update_partials(plk_ctx *ctx, const plk_op *ops, int nops);
+int plk_flush(plk_ctx *ctx);
+
+/* computeLikelihoodBranch (tree/phylokernelnew.h:2658-3184) for the branch joining the
+ * subtrees held by node references a and b (at most one may be a tip), length t.
+ * ptn_lh (nullable) receives per-pattern log-likelihoods (_pattern_lh, save_log_value). */
+int plk_branch_loglik(plk_ctx *ctx, int a, int b, double t, double *logl, double *ptn_lh);
+
+/* computeLikelihoodDerv (tree/phylokernelnew.h:2237-2650): first and second derivative of
+ * logL with respect to t.  reuse_theta != 0 reuses the device-resident theta buffer of
+ * the previous call on the same branch (PhyloTree::theta_computed). */
+int plk_branch_derivs(plk_ctx *ctx, int a, int b, double t, int reuse_theta, double *df,
+                      double *ddf);
+
+/* computeLikelihoodFromBuffer (tree/phylokernelnew.h:3284-3450): logL at length t from the
+ * theta buffer of the last plk_branch_derivs call. */
+int plk_loglik_from_theta(plk_ctx *ctx, double t, double *logl);
+
+/* Read back one slot: partial in canonical order [ptn][c][x] as FP64 (the FP32 form
+ * returns its own representation), and its scale counts (NORM: nptn uint16; SAFE:
+ * nptn*ncat uint16 [ptn][c]).  Either pointer may be NULL. */
+int plk_get_partial(plk_ctx *ctx, int slot, double *partial, uint16_t *scale);
+
+/* Kernel timing with HIP events recorded on the context's stream around every launch
+ * of the partial-traversal kernel.  kind: 0 traversal, 1 P(t), 2 derivative. */
+int plk_set_timing(plk_ctx *ctx, int enable);
+int plk_get_timing(plk_ctx *ctx, int kind, double *total_ms, int64_t *launches);
+
+/* Number of site-node partial updates (ops x patterns) executed since create. */
+int64_t plk_updates(plk_ctx *ctx);
+int plk_synchronize(plk_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+/*
+ * plk_tree.h -- PhyloTree-level host API of libplk (C ABI over the C++ PhyloTreeGPU).
+ *
+ * PhyloTreeGPU mirrors the part of IQ-TREE's PhyloTree that drives the likelihood
+ * kernels: the unrooted binary tree, directional partial bookkeeping with one slot per
+ * internal node (LM_PER_NODE, tree/phylotree.cpp:6011-6025), the post-order traversal
+ * planner (computeTraversalInfo, tree/phylotree.cpp:6031-6139), invalidation
+ * (clearReversePartialLh, tree/phylonode.cpp:22-33) and the Newton branch optimiser
+ * (optimizeOneBranch tree/phylotree.cpp:2620-2670, minimizeNewton
+ * utils/optimization.cpp:422-499, optimizeAllBranches tree/phylotree.cpp:2699-2801).
+ * Every kernel call goes through the plk.h C ABI.
+ *
+ * Nodes 0..ntaxa-1 are leaves (one neighbour, in slot 0 of nei), internal nodes are
+ * ntaxa..2*ntaxa-3; nei/len are (2*ntaxa-2) x 3 with -1 for unused slots.
+ */
+#ifndef PLK_TREE_H
+#define PLK_TREE_H
+
+#include <stdint.h>
+
+#include "plk.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct plk_tree plk_tree;
+
+int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling);
+int plk_tree_destroy(plk_tree *t);
+plk_ctx *plk_tree_context(plk_tree *t);
+int plk_tree_set_model(plk_tree *t, const double *eval, const double *evec, const double *inv_evec,
+                       const double *rates, const double *props, int state_unknown, const double *tip_lh);
+int plk_tree_set_patterns(plk_tree *t, const uint8_t *states, const double *freq, const double *invar);
+int plk_tree_set_topology(plk_tree *t, const int32_t *nei, const double *len);
+int plk_tree_set_length(plk_tree *t, int a, int b, double len);
+int plk_tree_get_length(plk_tree *t, int a, int b, double *len);
+int plk_tree_clear_all(plk_tree *t);
+
+/* computeLikelihoodBranch(dad_branch = dad->node, dad) (tree/phylokernelnew.h:2658) */
+int plk_tree_loglik(plk_tree *t, int dad, int node, double *logl, double *ptn_lh);
+/* computeLikelihood(): branch of the first leaf (tree/phylotree.cpp:1258-1299) */
+int plk_tree_compute_likelihood(plk_tree *t, double *logl);
+/* computeLikelihoodDerv at the branch's current length */
+int plk_tree_derivs(plk_tree *t, int dad, int node, int reuse_theta, double *df, double *ddf);
+/* computeLikelihoodFromBuffer at the branch's current length */
+int plk_tree_loglik_from_theta(plk_tree *t, int dad, int node, double *logl);
+int plk_tree_get_partial(plk_tree *t, int dad, int node, double *partial, uint16_t *scale);
+int plk_tree_optimize_one_branch(plk_tree *t, int node1, int node2, double min_bl, double max_bl,
+                                 int max_nr_step, double *optx);
+int plk_tree_optimize_all_branches(plk_tree *t, int iterations, double tol, double min_bl, double max_bl,
+                                   int max_nr_step, double *logl);
+/* Invalidate every partial and run one full traversal + branch logL (the benchmark step). */
+int plk_tree_full_traversal(plk_tree *t, double *logl);
+/* Counters: [0] partial updates (site x node), [1] derivative evaluations, [2] traversal ops */
+int plk_tree_counters(plk_tree *t, int64_t *out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

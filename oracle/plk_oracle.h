@@ -1,0 +1,92 @@
+/*
+ * plk_oracle.h -- CPU FP64 restatement of IQ-TREE's Felsenstein likelihood kernels.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP engine
+ * (tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may load
+ * it); the product path never links or calls it.
+ *
+ * It restates, in plain C with OpenMP, the reversible non-site-model kernels of
+ * tree/phylokernelnew.h:
+ *   computePartialInfo            tree/phylokernelnew.h:1013-1063
+ *   computeBounds                 tree/phylokernelnew.h:1116-1147
+ *   computeTraversalInfo          tree/phylotree.cpp:6031-6139, tree/phylokernelnew.h:1154-1285
+ *   computePartialLikelihoodSIMD  tree/phylokernelnew.h:1636-2036 (tip-tip, tip-internal,
+ *                                 internal-internal; NORM and SAFE scaling)
+ *   computeLikelihoodBranchSIMD   tree/phylokernelnew.h:2658-3184
+ *   computeLikelihoodBufferSIMD   tree/phylokernelnew.h:2052-2235
+ *   computeLikelihoodDervSIMD     tree/phylokernelnew.h:2237-2650
+ *   computeLikelihoodFromBufferSIMD tree/phylokernelnew.h:3284-3450
+ *   minimizeNewton                utils/optimization.cpp:422-499
+ *   optimizeOneBranch             tree/phylotree.cpp:2620-2670
+ *   optimizeAllBranches           tree/phylotree.cpp:2699-2801, tree/mtree.cpp:3040-3082
+ *   tip vectors                   tree/phylotreesse.cpp:354-375, model/modelsubst.cpp:183-208,
+ *                                 model/modeldna.cpp:462-476, model/modelprotein.cpp:1240-1256
+ *
+ * Memory layout follows the reference's pattern-interleaved SIMD layout with
+ * ORC_V lanes: element (ptn, c, x) of a partial sits at double offset
+ * (floor(ptn/V)*block + c*n + x)*V + ptn%V  (tree/phylokernelnew.h:1918-1920).
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   partials/logL    brute-force enumeration of internal states on 4-5 taxon trees,
+ *                    two-taxon closed form, pulley principle, pattern compression
+ *   scaling          deep caterpillar trees vs long-double brute force; NORM == SAFE
+ *   derivatives      Richardson finite differences of logL(t)
+ *   from-theta logL  equals branch logL at the same length
+ *   Newton           converges to the finite-difference optimum (df ~ 0, ddf < 0)
+ *   tip vectors      closed form Uinv * indicator(state set)
+ */
+#ifndef PLK_ORACLE_H
+#define PLK_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_tree orc_tree;
+
+/* sequence types for orc_tip_lh */
+enum { ORC_SEQ_DNA = 0, ORC_SEQ_PROTEIN = 1, ORC_SEQ_CODON = 2 };
+
+int orc_lanes(void);
+/* Number of directions (tips excluded) the oracle has recomputed since create. */
+long long orc_updates(const orc_tree *t);
+
+orc_tree *orc_create(int nstates, int ncat, int ntaxa, int nptn, int safe, int nthreads);
+void orc_destroy(orc_tree *t);
+
+void orc_set_model(orc_tree *t, const double *eval, const double *evec,
+                   const double *inv_evec, const double *rates, const double *props);
+/* tip_lh: (state_unknown+1) x nstates, already multiplied by the inverse eigenvectors */
+void orc_set_tip_lh(orc_tree *t, int state_unknown, const double *tip_lh);
+/* states: ntaxa x nptn (row per taxon); freq, invar: nptn */
+void orc_set_patterns(orc_tree *t, const uint8_t *states, const double *freq,
+                      const double *invar);
+/* nei, len: (2*ntaxa-2) x 3; leaves are nodes 0..ntaxa-1 with one neighbour (slot 0),
+ * unused slots are -1. */
+void orc_set_tree(orc_tree *t, const int *nei, const double *len);
+void orc_set_length(orc_tree *t, int a, int b, double len);
+double orc_get_length(const orc_tree *t, int a, int b);
+void orc_clear_all(orc_tree *t);
+
+double orc_branch_loglik(orc_tree *t, int dad, int node, double *ptn_lh);
+/* returns 0, or 1 when the derivative is not finite (NORM mode error of the reference) */
+int orc_branch_derivs(orc_tree *t, int dad, int node, int reuse_theta, double *df, double *ddf);
+double orc_loglik_from_theta(orc_tree *t, int dad, int node);
+/* partial of node's subtree seen from dad, canonical order [ptn][c][x]; scale is
+ * [ptn] (NORM) or [ptn][c] (SAFE).  Computes pending partials first. */
+int orc_get_partial(orc_tree *t, int dad, int node, double *out, uint16_t *scale);
+
+double orc_optimize_one_branch(orc_tree *t, int node1, int node2, double min_bl,
+                               double max_bl, int max_nr_step);
+double orc_optimize_all_branches(orc_tree *t, int iterations, double tol, double min_bl,
+                                 double max_bl, int max_nr_step);
+
+void orc_tip_lh(int nstates, int seq_type, const double *inv_evec, double *out);
+int orc_state_unknown(int nstates, int seq_type);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

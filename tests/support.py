@@ -1,0 +1,437 @@
+"""Test support: ctypes binding of the CPU oracle, independent numpy references
+(brute-force enumeration, real-space pruning with scipy expm), model builders and
+random trees.  Nothing here is used by the product path."""
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+import scipy.linalg
+import scipy.special
+import scipy.stats
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+ORACLE_DIR = ROOT / "oracle"
+
+SEQ_DNA, SEQ_PROTEIN, SEQ_CODON = 0, 1, 2
+LOG_SCALING_THRESHOLD = -177.4456782233459932741
+
+
+def _cpu_has_avx512():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return "avx512f" in line.split()
+    except OSError:
+        pass
+    return False
+
+
+_oracle = None
+
+
+def oracle_lib(lanes=None):
+    """Load the oracle shared library (built by oracle/Makefile)."""
+    global _oracle
+    if lanes is None and _oracle is not None:
+        return _oracle
+    if lanes is None:
+        lanes = 8 if _cpu_has_avx512() else 4
+    path = ORACLE_DIR / f"libplk_oracle_v{lanes}.so"
+    if not path.exists():
+        raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+    lib = ctypes.CDLL(str(path))
+    P = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    u16p = ctypes.POINTER(ctypes.c_uint16)
+    sig = {
+        "orc_lanes": (ctypes.c_int, []),
+        "orc_updates": (ctypes.c_longlong, [P]),
+        "orc_create": (P, [ctypes.c_int] * 6),
+        "orc_destroy": (None, [P]),
+        "orc_set_model": (None, [P, dp, dp, dp, dp, dp]),
+        "orc_set_tip_lh": (None, [P, ctypes.c_int, dp]),
+        "orc_set_patterns": (None, [P, u8p, dp, dp]),
+        "orc_set_tree": (None, [P, ip, dp]),
+        "orc_set_length": (None, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double]),
+        "orc_get_length": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_int]),
+        "orc_clear_all": (None, [P]),
+        "orc_branch_loglik": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_int, dp]),
+        "orc_branch_derivs": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp]),
+        "orc_loglik_from_theta": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_int]),
+        "orc_get_partial": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int, dp, u16p]),
+        "orc_optimize_one_branch": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_int]),
+        "orc_optimize_all_branches": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                        ctypes.c_double, ctypes.c_int]),
+        "orc_tip_lh": (None, [ctypes.c_int, ctypes.c_int, dp, dp]),
+        "orc_state_unknown": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if _oracle is None or lanes is None:
+        _oracle = lib
+    return lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Oracle:
+    """Thin object wrapper over one orc_tree."""
+
+    def __init__(self, prob, safe=False, nthreads=2, lanes=None):
+        self.lib = oracle_lib(lanes)
+        self.p = prob
+        n, ncat = prob.n, len(prob.rates)
+        self.h = self.lib.orc_create(n, ncat, prob.ntaxa, prob.nptn, int(safe), nthreads)
+        self._keep = []
+        m = prob
+        self.lib.orc_set_model(self.h, _dp(m.eval), _dp(m.evec), _dp(m.inv_evec), _dp(m.rates), _dp(m.props))
+        su = self.lib.orc_state_unknown(n, m.seq_type)
+        tip = np.zeros((su + 1) * n)
+        self.lib.orc_tip_lh(n, m.seq_type, _dp(m.inv_evec), _dp(tip))
+        self.tip_lh = tip.reshape(su + 1, n)
+        self.lib.orc_set_tip_lh(self.h, su, _dp(tip))
+        states = np.ascontiguousarray(m.states, dtype=np.uint8)
+        freq = np.ascontiguousarray(m.freq, dtype=np.float64)
+        invar = np.ascontiguousarray(m.invar if m.invar is not None else np.zeros(m.nptn))
+        self.lib.orc_set_patterns(self.h, states.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  _dp(freq), _dp(invar))
+        nei = np.ascontiguousarray(m.nei, dtype=np.int32)
+        blen = np.ascontiguousarray(m.blen, dtype=np.float64)
+        self.lib.orc_set_tree(self.h, nei.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _dp(blen))
+
+    def close(self):
+        if self.h:
+            self.lib.orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def loglik(self, a, b, with_patterns=False):
+        pl = np.zeros(self.p.nptn)
+        v = self.lib.orc_branch_loglik(self.h, a, b, _dp(pl))
+        return (v, pl) if with_patterns else v
+
+    def derivs(self, a, b, reuse=False):
+        df = np.zeros(1)
+        ddf = np.zeros(1)
+        st = self.lib.orc_branch_derivs(self.h, a, b, int(reuse), _dp(df), _dp(ddf))
+        return float(df[0]), float(ddf[0]), st
+
+    def loglik_from_theta(self, a, b):
+        return self.lib.orc_loglik_from_theta(self.h, a, b)
+
+    def set_length(self, a, b, t):
+        self.lib.orc_set_length(self.h, a, b, t)
+
+    def get_length(self, a, b):
+        return self.lib.orc_get_length(self.h, a, b)
+
+    def partial(self, dad, node, safe=False):
+        n, ncat = self.p.n, len(self.p.rates)
+        out = np.zeros(self.p.nptn * n * ncat)
+        sc = np.zeros(self.p.nptn * (ncat if safe else 1), dtype=np.uint16)
+        st = self.lib.orc_get_partial(self.h, dad, node, _dp(out), sc.ctypes.data_as(ctypes.POINTER(ctypes.c_uint16)))
+        assert st == 0
+        return out.reshape(self.p.nptn, ncat, n), sc
+
+    def optimize_all(self, iterations=100, tol=1e-3, min_bl=1e-6, max_bl=10.0, max_nr=100):
+        return self.lib.orc_optimize_all_branches(self.h, iterations, tol, min_bl, max_bl, max_nr)
+
+    def optimize_one(self, a, b, min_bl=1e-6, max_bl=10.0, max_nr=100):
+        return self.lib.orc_optimize_one_branch(self.h, a, b, min_bl, max_bl, max_nr)
+
+
+# ---------------------------------------------------------------------------------------
+# numpy model builders (independent of the product's C++ host code)
+
+
+def rate_matrix(exch, pi):
+    """Q = R diag(pi), zero row sums, normalised to one substitution per unit time
+    (model/modelmarkov.cpp:1472-1514).  exch is the upper-triangle row-major list."""
+    n = len(pi)
+    R = np.zeros((n, n))
+    iu = np.triu_indices(n, 1)
+    R[iu] = exch
+    R = R + R.T
+    Q = R * pi[None, :]
+    np.fill_diagonal(Q, 0.0)
+    rs = Q.sum(axis=1)
+    Q -= np.diag(rs)
+    Q /= np.dot(rs, pi)
+    return Q
+
+
+def eigen_system(Q, pi):
+    """Symmetrised eigendecomposition (model/modelmarkov.cpp:1519-1562)."""
+    s = np.sqrt(pi)
+    S = (s[:, None] * Q) / s[None, :]
+    S = 0.5 * (S + S.T)
+    w, Vm = np.linalg.eigh(S)
+    evec = Vm / s[:, None]
+    inv_evec = Vm.T * s[None, :]
+    return w, np.ascontiguousarray(evec), np.ascontiguousarray(inv_evec)
+
+
+def gamma_rates_exact(alpha, k):
+    """Yang (1994) mean-of-category discrete gamma rates, exact via scipy."""
+    cuts = scipy.stats.gamma.ppf(np.arange(1, k) / k, a=alpha, scale=1.0 / alpha)
+    upper = scipy.special.gammainc(alpha + 1, cuts * alpha)
+    edges = np.concatenate([[0.0], upper, [1.0]])
+    rates = np.diff(edges) * k
+    return rates * k / rates.sum()
+
+
+def codon_exch(kappa, omega):
+    """GY-style codon exchangeabilities over the 61 sense codons of the standard code,
+    codons in AAA..TTT order (model/modelcodon.cpp:366-387, 574-651, 877-906)."""
+    code = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF"
+    sense = [c for c in range(64) if code[c] != "*"]
+    n = len(sense)
+    R = np.zeros((n, n))
+    for a in range(n):
+        for b in range(a + 1, n):
+            ci, cj = sense[a], sense[b]
+            d = [(ci >> s) & 3 != (cj >> s) & 3 for s in (4, 2, 0)]
+            if sum(d) != 1:
+                continue
+            pos = [4, 2, 0][d.index(True)]
+            n1, n2 = (ci >> pos) & 3, (cj >> pos) & 3
+            ts = abs(n1 - n2) == 2
+            syn = code[ci] == code[cj]
+            r = 1.0
+            if syn:
+                if ts:
+                    r *= kappa
+            else:
+                r *= omega * kappa if ts else omega
+            R[a, b] = r
+    return R[np.triu_indices(n, 1)], sense
+
+
+class Problem:
+    """A complete likelihood problem: model, tree, patterns."""
+
+    def __init__(self, **kw):
+        self.__dict__.update(kw)
+
+
+def random_tree(ntaxa, rng, mean_len=0.1):
+    """Random unrooted binary tree by repeatedly splitting a leaf edge; nodes 0..T-1 leaves."""
+    nn = 2 * ntaxa - 2
+    nei = -np.ones((nn, 3), dtype=np.int32)
+    blen = np.zeros((nn, 3))
+    edges = []
+
+    def link(a, b, t):
+        ka = list(nei[a]).index(-1)
+        kb = list(nei[b]).index(-1)
+        nei[a, ka] = b
+        nei[b, kb] = a
+        blen[a, ka] = t
+        blen[b, kb] = t
+
+    c = ntaxa
+    nxt_int = ntaxa + 1
+    for leaf in range(3):
+        link(c, leaf, rng.exponential(mean_len))
+        edges.append((c, leaf))
+    for leaf in range(3, ntaxa):
+        a, b = edges[rng.integers(len(edges))]
+        m = nxt_int
+        nxt_int += 1
+        # split edge (a,b) with new internal node m, attach leaf
+        ka = list(nei[a]).index(b)
+        kb = list(nei[b]).index(a)
+        nei[a, ka] = -1
+        nei[b, kb] = -1
+        edges.remove((a, b))
+        link(a, m, rng.exponential(mean_len))
+        link(m, b, rng.exponential(mean_len))
+        link(m, leaf, rng.exponential(mean_len))
+        edges += [(a, m), (m, b), (m, leaf)]
+    # leaves keep their single neighbour in slot 0
+    for leaf in range(ntaxa):
+        k = [i for i in range(3) if nei[leaf, i] >= 0][0]
+        if k != 0:
+            nei[leaf, 0], nei[leaf, k] = nei[leaf, k], -1
+            blen[leaf, 0], blen[leaf, k] = blen[leaf, k], 0.0
+    return nei, np.maximum(blen, 1e-6) * (nei >= 0)
+
+
+def caterpillar_tree(ntaxa, length):
+    nn = 2 * ntaxa - 2
+    nei = -np.ones((nn, 3), dtype=np.int32)
+    blen = np.zeros((nn, 3))
+
+    def link(a, b):
+        ka = list(nei[a]).index(-1)
+        kb = list(nei[b]).index(-1)
+        nei[a, ka], nei[b, kb] = b, a
+        blen[a, ka] = blen[b, kb] = length
+
+    internals = list(range(ntaxa, nn))
+    link(internals[0], 0)
+    link(internals[0], 1)
+    for i in range(len(internals) - 1):
+        link(internals[i], internals[i + 1])
+        link(internals[i + 1], i + 2)
+    link(internals[-1], ntaxa - 1)
+    return nei, blen
+
+
+def make_problem(n, ntaxa, nsites, rng, seq_type=None, alpha=0.5, ncat=4, mean_len=0.1,
+                 ambiguous=0.0, tree=None, compress=True):
+    if seq_type is None:
+        seq_type = {4: SEQ_DNA, 20: SEQ_PROTEIN}.get(n, SEQ_CODON)
+    pi = rng.dirichlet(np.ones(n))
+    if n == 4:
+        exch = rng.uniform(0.5, 2.0, 6)
+    elif n == 20:
+        exch = rng.gamma(1.0, 1.0, 190)
+    else:
+        exch, _ = codon_exch(rng.uniform(1, 4), rng.uniform(0.1, 1))
+    Q = rate_matrix(exch, pi)
+    w, U, Ui = eigen_system(Q, pi)
+    rates = gamma_rates_exact(alpha, ncat) if ncat > 1 else np.ones(1)
+    props = np.full(ncat, 1.0 / ncat)
+    if tree is None:
+        nei, blen = random_tree(ntaxa, rng, mean_len)
+    else:
+        nei, blen = tree
+    # simulate i.i.d. random states (not down the tree: tests want arbitrary patterns)
+    states = rng.integers(0, n, size=(ntaxa, nsites)).astype(np.uint8)
+    su = {SEQ_DNA: 18, SEQ_PROTEIN: 23}.get(seq_type, n)
+    if ambiguous > 0:
+        mask = rng.random(states.shape) < ambiguous
+        if seq_type == SEQ_DNA:
+            amb = rng.integers(n, su + 1, size=states.shape)
+        elif seq_type == SEQ_PROTEIN:
+            amb = rng.integers(n, su + 1, size=states.shape)
+        else:
+            amb = np.full(states.shape, su)
+        states = np.where(mask, amb, states).astype(np.uint8)
+    if compress:
+        cols, inv, cnt = np.unique(states.T, axis=0, return_inverse=True, return_counts=True)
+        states = np.ascontiguousarray(cols.T)
+        freq = cnt.astype(np.float64)
+    else:
+        freq = np.ones(nsites)
+    return Problem(n=n, ntaxa=ntaxa, nptn=states.shape[1], seq_type=seq_type, pi=pi, Q=Q,
+                   eval=np.ascontiguousarray(w), evec=U, inv_evec=Ui, rates=np.ascontiguousarray(rates),
+                   props=props, nei=nei, blen=blen, states=states, freq=freq, invar=None, su=su)
+
+
+def tip_likelihood(prob, s):
+    """Real-space tip vector of a state (independent of the oracle)."""
+    n = prob.n
+    v = np.zeros(n)
+    if s < n:
+        v[s] = 1.0
+    elif prob.seq_type == SEQ_DNA and s < 18:
+        for j in range(n):
+            if (s - n + 1) & (1 << j):
+                v[j] = 1.0
+    elif prob.seq_type == SEQ_PROTEIN and s < n + 3:
+        amb = [(2, 3), (5, 6), (9, 10)][s - n]
+        v[list(amb)] = 1.0
+    else:
+        v[:] = 1.0
+    return v
+
+
+def pruning_loglik(prob, a, b, dtype=np.float64):
+    """Independent real-space Felsenstein pruning with scipy expm and per-node
+    renormalisation (log accumulation); returns per-pattern logL and total."""
+    nei, blen = prob.nei, prob.blen
+    ncat = len(prob.rates)
+    P = {}
+
+    def pmat(t, c):
+        key = (round(t, 17), c)
+        if key not in P:
+            P[key] = scipy.linalg.expm(prob.Q * prob.rates[c] * t)
+        return P[key]
+
+    def length(u, v):
+        k = list(nei[u]).index(v)
+        return blen[u, k]
+
+    def down(node, dad):
+        # returns (L[c][ptn][x], logscale[ptn])
+        if node < prob.ntaxa:
+            T = np.stack([tip_likelihood(prob, s) for s in prob.states[node]])  # ptn x n
+            return np.repeat(T[None], ncat, axis=0), np.zeros(prob.nptn)
+        acc = np.ones((ncat, prob.nptn, prob.n))
+        lsc = np.zeros(prob.nptn)
+        for k in range(3):
+            ch = nei[node, k]
+            if ch < 0 or ch == dad:
+                continue
+            Lc, sc = down(ch, node)
+            t = blen[node, k]
+            for c in range(ncat):
+                acc[c] *= Lc[c] @ pmat(t, c).T
+            lsc += sc
+        m = acc.max(axis=(0, 2))
+        m[m == 0] = 1.0
+        acc /= m[None, :, None]
+        return acc, lsc + np.log(m)
+
+    La, sa = down(a, b)
+    Lb, sb = down(b, a)
+    t = length(a, b)
+    lh = np.zeros(prob.nptn)
+    for c in range(ncat):
+        lh += prob.props[c] * np.einsum("px,px->p", La[c] * prob.pi[None, :], Lb[c] @ pmat(t, c).T)
+    lp = np.log(lh) + sa + sb
+    return lp, float(np.dot(lp, prob.freq))
+
+
+def brute_force_loglik(prob, a, b):
+    """Enumerate all internal-state assignments (tiny trees only).  Edges are oriented
+    away from an internal root: L = sum pi_root * prod P[parent][child]."""
+    import itertools
+    ntaxa, n = prob.ntaxa, prob.n
+    nn = 2 * ntaxa - 2
+    internals = list(range(ntaxa, nn))
+    root = internals[0]
+    directed = []
+    stack = [(root, -1)]
+    while stack:
+        u, par = stack.pop()
+        for k in range(3):
+            v = prob.nei[u, k]
+            if v < 0 or v == par:
+                continue
+            directed.append((u, v, prob.blen[u, k]))
+            if v >= ntaxa:
+                stack.append((v, u))
+    ncat = len(prob.rates)
+    tips = np.stack([[tip_likelihood(prob, s) for s in prob.states[i]] for i in range(ntaxa)])  # T x ptn x n
+    total = np.zeros(prob.nptn)
+    for c in range(ncat):
+        Pm = {(u, v): scipy.linalg.expm(prob.Q * prob.rates[c] * t) for (u, v, t) in directed}
+        lc = np.zeros(prob.nptn)
+        for assign in itertools.product(range(n), repeat=len(internals)):
+            st = dict(zip(internals, assign))
+            term = np.full(prob.nptn, prob.pi[st[root]])
+            for (u, v, t) in directed:
+                M = Pm[(u, v)]
+                if v >= ntaxa:
+                    term = term * M[st[u], st[v]]
+                else:
+                    term = term * (tips[v] @ M[st[u]])
+            lc += term
+        total += prob.props[c] * lc
+    lp = np.log(total)
+    return lp, float(np.dot(lp, prob.freq))

@@ -1,0 +1,171 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle on the same
+seeded inputs.  Tolerance: |dlogL|/|logL| <= 1e-10 (FP64 form), scale counts bit-exact."""
+import sys
+
+import numpy as np
+import pytest
+
+import gems_amd as G
+import support as S
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def branches(p, limit=None):
+    out = []
+    for u in range(2 * p.ntaxa - 2):
+        for k in range(3):
+            v = int(p.nei[u, k])
+            if v > u:
+                out.append((u, v))
+    return out[:limit] if limit else out
+
+
+def rel(a, b):
+    return abs(a - b) / max(abs(b), 1e-300)
+
+
+CASES = [
+    (G.SEQ_DNA, 32, 10000, 0.1, 0.5),     # DNA GTR+G4, 32 taxa, 10k sites
+    (G.SEQ_PROTEIN, 24, 3000, 0.2, 0.8),
+    (G.SEQ_CODON, 12, 600, 0.1, 0.5),
+]
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+@pytest.mark.parametrize("case", CASES, ids=["dna", "protein", "codon"])
+def test_loglik_parity(case, safe):
+    seq, T, sites, ml, alpha = case
+    say("case", case, "safe", safe)
+    p = G.Problem(seq, T, sites, mean_len=ml, alpha=alpha, seed=11)
+    g = G.Tree(p, scaling=safe)
+    o = S.Oracle(p, safe=bool(safe), nthreads=4)
+    for (u, v) in branches(p, 6) + [(0, int(p.nei[0, 0]))]:
+        lg, plg = g.loglik(u, v, with_patterns=True)
+        lo, plo = o.loglik(u, v, with_patterns=True)
+        assert rel(lg, lo) <= TOL, (u, v, lg, lo)
+        np.testing.assert_allclose(plg, plo, rtol=1e-10, atol=1e-10)
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+def test_partials_and_scale_counts(safe):
+    T = 150
+    p = G.Problem(G.SEQ_DNA, T, 700, mean_len=0.6, alpha=0.5, seed=3)
+    say("partials safe", safe, "nptn", p.nptn)
+    g = G.Tree(p, scaling=safe)
+    o = S.Oracle(p, safe=bool(safe))
+    a, b = 0, int(p.nei[0, 0])
+    assert rel(g.loglik(a, b), o.loglik(a, b)) <= TOL
+    # several internal directions
+    checked = 0
+    for v in range(T, 2 * T - 2, 17):
+        dad = int(p.nei[v, 0])
+        pg, sg = g.partial(dad, v, safe=bool(safe))
+        po, so = o.partial(dad, v, safe=bool(safe))
+        assert np.array_equal(sg, so)
+        scale = np.abs(po).max(axis=(1, 2), keepdims=True)
+        assert np.all(np.abs(pg - po) <= 1e-12 * scale + 1e-300)
+        checked += 1
+    assert checked > 3
+
+
+@pytest.mark.parametrize("safe", [0, 1])
+def test_forced_scaling_caterpillar(safe):
+    T = 300
+    rng = np.random.default_rng(5)
+    pp = S.make_problem(4, T, 150, rng, tree=S.caterpillar_tree(T, 1.5), ambiguous=0.05)
+    say("caterpillar safe", safe)
+    g = G.Tree(pp, scaling=safe)
+    o = S.Oracle(pp, safe=bool(safe))
+    a, b = 0, int(pp.nei[0, 0])
+    lg, lo = g.loglik(a, b), o.loglik(a, b)
+    assert rel(lg, lo) <= TOL
+    _, sg = g.partial(a, b, safe=bool(safe))
+    _, so = o.partial(a, b, safe=bool(safe))
+    assert so.max() >= 1 and np.array_equal(sg, so)
+    df_g, ddf_g, _ = g.derivs(a, b)
+    df_o, ddf_o, _ = o.derivs(a, b)
+    assert abs(df_g - df_o) <= 1e-9 * max(1, abs(df_o))
+    assert abs(ddf_g - ddf_o) <= 1e-9 * max(1, abs(ddf_o))
+
+
+@pytest.mark.parametrize("case", CASES, ids=["dna", "protein", "codon"])
+def test_derivatives_and_theta(case):
+    seq, T, sites, ml, alpha = case
+    say("derivs", case)
+    p = G.Problem(seq, T, min(sites, 3000), mean_len=ml, alpha=alpha, seed=5)
+    safe = 1 if seq == G.SEQ_CODON else 0
+    g = G.Tree(p, scaling=safe)
+    o = S.Oracle(p, safe=bool(safe))
+    for (u, v) in branches(p, 5):
+        dg, ddg, rc = g.derivs(u, v)
+        assert rc == 0
+        do, ddo, _ = o.derivs(u, v)
+        assert abs(dg - do) <= 1e-9 * max(1.0, abs(do)), (dg, do)
+        assert abs(ddg - ddo) <= 1e-9 * max(1.0, abs(ddo)), (ddg, ddo)
+        # Newton re-evaluation from the resident theta buffer at a new length
+        t0 = g.get_length(u, v)
+        for t in (t0 * 0.5, t0 * 2 + 0.01):
+            g.set_length(u, v, t)
+            o.set_length(u, v, t)
+            dg2, ddg2, _ = g.derivs(u, v, reuse=True)
+            do2, ddo2, _ = o.derivs(u, v, reuse=True)
+            assert abs(dg2 - do2) <= 1e-9 * max(1.0, abs(do2))
+            assert rel(g.loglik_from_theta(u, v), o.loglik_from_theta(u, v)) <= TOL
+
+
+@pytest.mark.parametrize("nsites", [1, 2, 63, 64, 65, 130])
+def test_ragged_pattern_counts(nsites):
+    rng = np.random.default_rng(nsites)
+    pp = S.make_problem(20, 9, nsites, rng, ambiguous=0.2, compress=False)
+    say("ragged", nsites)
+    g = G.Tree(pp)
+    o = S.Oracle(pp)
+    for (u, v) in branches(pp, 4):
+        assert rel(g.loglik(u, v), o.loglik(u, v)) <= TOL
+
+
+def test_newton_optimize_all_parity():
+    p = G.Problem(G.SEQ_DNA, 16, 3000, seed=9)
+    say("newton")
+    g = G.Tree(p)
+    o = S.Oracle(p)
+    lg = g.optimize_all(iterations=5, tol=1e-6)
+    lo = o.optimize_all(iterations=5, tol=1e-6)
+    assert rel(lg, lo) <= 1e-9
+    for (u, v) in branches(p):
+        assert abs(g.get_length(u, v) - o.get_length(u, v)) <= 1e-7 * max(1e-3, o.get_length(u, v))
+
+
+def test_deterministic_repeat():
+    p = G.Problem(G.SEQ_PROTEIN, 30, 4000, seed=2)
+    g = G.Tree(p)
+    a = g.full_traversal()
+    b = g.full_traversal()
+    assert a == b
+
+
+def test_invalid_ops_rejected():
+    import ctypes
+    p = G.Problem(G.SEQ_DNA, 8, 100, seed=1)
+    g = G.Tree(p)
+    ctx = g.context()
+    op = (G.plk_op * 1)()
+    op[0].out_slot = 99
+    op[0].child[0] = -1
+    op[0].child[1] = -2
+    rc = G.lib().plk_update_partials(ctx, op, 1)
+    assert rc == -1
+    v = ctypes.c_double()
+    assert G.lib().plk_branch_loglik(ctx, -1, -2, 0.1, ctypes.byref(v), None) == -1
+    assert G.lib().plk_branch_loglik(ctx, 0, 1000, 0.1, ctypes.byref(v), None) == -1
+    # the context still works afterwards
+    o = S.Oracle(p)
+    assert rel(g.loglik(0, int(p.nei[0, 0])), o.loglik(0, int(p.nei[0, 0]))) <= TOL

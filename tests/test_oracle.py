@@ -1,0 +1,181 @@
+"""Pins the CPU oracle (oracle/plk_oracle.c) against checks that do not share its
+arithmetic: brute-force enumeration, real-space pruning with scipy.linalg.expm,
+closed forms, invariants and finite differences."""
+import numpy as np
+import pytest
+
+import support as S
+
+
+def branches(p):
+    out = []
+    for u in range(2 * p.ntaxa - 2):
+        for k in range(3):
+            v = int(p.nei[u, k])
+            if v > u:
+                out.append((u, v))
+    return out
+
+
+@pytest.mark.parametrize("n,T,sites", [(4, 4, 300), (4, 5, 300), (20, 4, 60), (20, 5, 30)])
+def test_brute_force(n, T, sites):
+    rng = np.random.default_rng(n * 100 + T)
+    p = S.make_problem(n, T, sites, rng, ambiguous=0.15)
+    o = S.Oracle(p)
+    a, b = 0, int(p.nei[0, 0])
+    lp_bf, tot_bf = S.brute_force_loglik(p, a, b)
+    tot, lp = o.loglik(a, b, with_patterns=True)
+    assert abs(tot - tot_bf) <= 1e-12 * abs(tot_bf)
+    np.testing.assert_allclose(lp, lp_bf, rtol=1e-12, atol=1e-12)
+
+
+def test_codon_pruning():
+    rng = np.random.default_rng(7)
+    p = S.make_problem(61, 6, 40, rng)
+    o = S.Oracle(p, safe=True)
+    a, b = 0, int(p.nei[0, 0])
+    _, ref = S.pruning_loglik(p, a, b)
+    assert abs(o.loglik(a, b) - ref) <= 1e-11 * abs(ref)
+
+
+@pytest.mark.parametrize("n", [4, 20])
+def test_pulley_and_pruning(n):
+    rng = np.random.default_rng(3 + n)
+    p = S.make_problem(n, 12, 400, rng, ambiguous=0.05)
+    o = S.Oracle(p)
+    vals = [o.loglik(u, v) for (u, v) in branches(p)]
+    _, ref = S.pruning_loglik(p, 0, int(p.nei[0, 0]))
+    assert np.ptp(vals) <= 1e-12 * abs(ref)
+    assert abs(vals[0] - ref) <= 1e-12 * abs(ref)
+
+
+def test_three_taxon_closed_form():
+    # star tree: L = sum_c w_c sum_x pi_x prod_leaf P_c(t_leaf)[x, s_leaf]
+    import scipy.linalg
+    rng = np.random.default_rng(11)
+    p = S.make_problem(4, 3, 50, rng)
+    o = S.Oracle(p)
+    lp = np.zeros(p.nptn)
+    center = 3
+    for c in range(len(p.rates)):
+        acc = np.tile(p.pi, (p.nptn, 1))
+        for leaf in range(3):
+            t = p.blen[leaf, 0]
+            P = scipy.linalg.expm(p.Q * p.rates[c] * t)
+            acc *= P[:, p.states[leaf]].T
+        lp += p.props[c] * acc.sum(axis=1)
+    ref = float(np.dot(np.log(lp), p.freq))
+    assert abs(o.loglik(center, 0) - ref) <= 1e-12 * abs(ref)
+
+
+def test_compression_invariance():
+    rng = np.random.default_rng(5)
+    p1 = S.make_problem(4, 6, 500, np.random.default_rng(5), compress=False)
+    p2 = S.make_problem(4, 6, 500, np.random.default_rng(5), compress=True)
+    assert p2.nptn < p1.nptn
+    l1 = S.Oracle(p1).loglik(0, int(p1.nei[0, 0]))
+    l2 = S.Oracle(p2).loglik(0, int(p2.nei[0, 0]))
+    assert abs(l1 - l2) <= 1e-11 * abs(l1)
+
+
+@pytest.mark.parametrize("n", [4, 20])
+def test_scaling_deep_tree(n):
+    """A long caterpillar forces 2^-256 rescaling many times; logL must equal the
+    renormalising real-space pruning, and NORM must equal SAFE."""
+    T = 300 if n == 4 else 100
+    rng = np.random.default_rng(17)
+    p = S.make_problem(n, T, 80, rng, tree=S.caterpillar_tree(T, 1.5))
+    on = S.Oracle(p, safe=False)
+    os_ = S.Oracle(p, safe=True)
+    a, b = 0, int(p.nei[0, 0])
+    ln = on.loglik(a, b)
+    ls = os_.loglik(a, b)
+    _, ref = S.pruning_loglik(p, a, b)
+    assert abs(ln - ref) <= 1e-11 * abs(ref)
+    assert abs(ls - ref) <= 1e-11 * abs(ref)
+    # scaling really happened
+    part, sc = on.partial(a, b)
+    assert sc.max() >= 1
+    part_s, sc_s = os_.partial(a, b, safe=True)
+    assert sc_s.max() >= 1
+
+
+@pytest.mark.parametrize("n", [4, 20])
+def test_derivatives_finite_difference(n):
+    rng = np.random.default_rng(23 + n)
+    p = S.make_problem(n, 8, 300, rng)
+    o = S.Oracle(p)
+    for (u, v) in branches(p)[:4]:
+        t0 = o.get_length(u, v)
+        df, ddf, st = o.derivs(u, v)
+        assert st == 0
+        h = 1e-4 * max(t0, 1e-3)
+
+        def f(t):
+            o.set_length(u, v, t)
+            return o.loglik(u, v)
+
+        # Richardson-extrapolated central differences
+        d1h = (f(t0 + h) - f(t0 - h)) / (2 * h)
+        d1h2 = (f(t0 + h / 2) - f(t0 - h / 2)) / h
+        d1 = (4 * d1h2 - d1h) / 3
+        f0 = f(t0)
+        d2h = (f(t0 + h) - 2 * f0 + f(t0 - h)) / h ** 2
+        d2h2 = (f(t0 + h / 2) - 2 * f0 + f(t0 - h / 2)) / (h / 2) ** 2
+        d2 = (4 * d2h2 - d2h) / 3
+        o.set_length(u, v, t0)
+        assert abs(df - d1) <= 1e-6 * max(1.0, abs(df))
+        assert abs(ddf - d2) <= 1e-4 * max(1.0, abs(ddf))
+
+
+def test_from_theta_equals_branch():
+    rng = np.random.default_rng(29)
+    p = S.make_problem(20, 7, 100, rng)
+    o = S.Oracle(p)
+    u, v = branches(p)[3]
+    o.derivs(u, v)
+    for t in (0.01, 0.3, 2.0):
+        o.lib.orc_set_length(o.h, u, v, t)
+        # from-theta uses the current length; the theta buffer stays valid for this branch
+        lt = o.loglik_from_theta(u, v)
+        lb = o.loglik(u, v)
+        assert abs(lt - lb) <= 1e-12 * abs(lb)
+        o.derivs(u, v)
+
+
+def test_newton_reaches_stationary_point():
+    rng = np.random.default_rng(31)
+    p = S.make_problem(4, 10, 800, rng)
+    o = S.Oracle(p)
+    l0 = o.loglik(0, int(p.nei[0, 0]))
+    l1 = o.optimize_all(iterations=20, tol=1e-6)
+    assert l1 >= l0
+    for (u, v) in branches(p):
+        t = o.get_length(u, v)
+        df, ddf, _ = o.derivs(u, v)
+        if 1e-5 < t < 9.0:
+            assert abs(df) < 0.05, (u, v, t, df)
+
+
+def test_tip_vectors_closed_form():
+    rng = np.random.default_rng(2)
+    for n, st in [(4, S.SEQ_DNA), (20, S.SEQ_PROTEIN)]:
+        p = S.make_problem(n, 4, 10, rng, seq_type=st)
+        o = S.Oracle(p)
+        for s in range(o.tip_lh.shape[0]):
+            ref = p.inv_evec @ S.tip_likelihood(p, s)
+            np.testing.assert_allclose(o.tip_lh[s], ref, rtol=0, atol=1e-14)
+
+
+def test_planted_mistake_is_caught():
+    """Transposing P in the independent pruning reference must break agreement: the
+    checks above are sensitive to an operand-order mistake."""
+    rng = np.random.default_rng(41)
+    p = S.make_problem(4, 5, 100, rng)
+    o = S.Oracle(p)
+    a, b = 0, int(p.nei[0, 0])
+    good = o.loglik(a, b)
+    bad = p.__class__(**dict(p.__dict__))
+    bad.Q = p.Q.T.copy()  # wrong orientation of the rate matrix
+    _, ref_bad = S.pruning_loglik(bad, a, b)
+    assert abs(good - ref_bad) > 1e-6 * abs(good)
