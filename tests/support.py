@@ -326,9 +326,11 @@ def make_problem(n, ntaxa, nsites, rng, seq_type=None, alpha=0.5, ncat=4, mean_l
         freq = cnt.astype(np.float64)
     else:
         freq = np.ones(nsites)
-    return Problem(n=n, ntaxa=ntaxa, nptn=states.shape[1], seq_type=seq_type, pi=pi, Q=Q,
+    prob = Problem(n=n, ntaxa=ntaxa, nptn=states.shape[1], seq_type=seq_type, pi=pi, Q=Q,
                    eval=np.ascontiguousarray(w), evec=U, inv_evec=Ui, rates=np.ascontiguousarray(rates),
                    props=props, nei=nei, blen=blen, states=states, freq=freq, invar=None, su=su)
+    prob.tip_lh = np.stack([Ui @ tip_likelihood(prob, s) for s in range(su + 1)])
+    return prob
 
 
 def tip_likelihood(prob, s):
