@@ -10,6 +10,8 @@
 // no grid-wide barrier; each thread only re-reads values it wrote itself.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "engine_common.h"
 
 namespace plk {
@@ -20,6 +22,33 @@ constexpr double kTwo256 = 0x1p256;
 constexpr double kLogST = -177.4456782233459932741;
 
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+
+// Read-only view in the constant address space: wave-uniform reads through it compile to
+// scalar loads whose results feed the FMAs as SGPR operands (the matrices are written
+// by an earlier launch only).
+typedef const __attribute__((address_space(4))) double *cdptr;
+__device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
+
+// Compile-time loop: f(I) for I = B..E-1, fully unrolled.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// x-loop over output states: unrolled for small state counts, a plain loop for the
+// 61-state codon model (whose fast path is the FP32 matrix-core form).
+template <int N, typename F>
+__device__ __forceinline__ void for_states(F &&f) {
+    if constexpr (N <= 20) {
+        static_for<0, N>([&](auto X) { f(decltype(X)::value); });
+    } else {
+#pragma unroll 1
+        for (int x = 0; x < N; x++) f(x);
+    }
+}
 
 // Deterministic wavefront sum (fixed shuffle tree); result valid in lane 0.
 __device__ __forceinline__ double wave_sum(double v) {
@@ -100,11 +129,10 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
                                            int64_t off, int sl, int sr, double (&out)[N]) {
     const int ncat = P.ncat, block = N * ncat;
     const double *__restrict__ part = (const double *)P.part;
-    const double *__restrict__ uinv = P.inv_evec;
-    const double *__restrict__ eL = M + (int64_t)c * N * N;
-    const double *__restrict__ eR = M + (int64_t)(ncat + c) * N * N;
+    const cdptr uinv = as_const(P.inv_evec);
+    const cdptr eL = as_const(M + (int64_t)c * N * N);
+    const cdptr eR = as_const(M + (int64_t)(ncat + c) * N * N);
     const double *__restrict__ tabs = M + (int64_t)2 * ncat * N * N;
-    constexpr int UX = (N <= 20) ? N : 1;
 #pragma unroll
     for (int j = 0; j < N; j++) out[j] = 0.0;
     if (op.kind == KIND_II) {
@@ -116,8 +144,7 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
             l[i] = L[i * 64];
             r[i] = R[i * 64];
         }
-#pragma unroll UX
-        for (int x = 0; x < N; x++) {
+        for_states<N>([&](const int x) {
             double a = 0.0, b = 0.0;
 #pragma unroll
             for (int i = 0; i < N; i++) {
@@ -127,31 +154,29 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
             const double t = a * b;
 #pragma unroll
             for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);
-        }
+        });
     } else if (op.kind == KIND_TI) {
         const double *__restrict__ R = part + (int64_t)op.right * P.slot_stride + off;
         const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
         double r[N];
 #pragma unroll
         for (int i = 0; i < N; i++) r[i] = R[i * 64];
-#pragma unroll UX
-        for (int x = 0; x < N; x++) {
+        for_states<N>([&](const int x) {
             double b = 0.0;
 #pragma unroll
             for (int i = 0; i < N; i++) b = fma(eR[x * N + i], r[i], b);
             const double t = tl[x] * b;
 #pragma unroll
             for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);
-        }
+        });
     } else {
         const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
         const double *__restrict__ tr = tabs + (int64_t)(P.su + 1) * block + (int64_t)sr * block + c * N;
-#pragma unroll UX
-        for (int x = 0; x < N; x++) {
+        for_states<N>([&](const int x) {
             const double t = tl[x] * tr[x];
 #pragma unroll
             for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);
-        }
+        });
     }
 }
 
