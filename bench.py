@@ -82,55 +82,73 @@ def main():
     n_ops = p.ntaxa - 2
     upd_per_step = n_ops * p.nptn
 
-    g = G.Tree(p, precision=G.FP64, scaling=G.NORM)
-    for _ in range(args.warmup):
-        g.full_traversal()
-    g.set_timing(True)
-    G.lib().plk_synchronize(g.context())
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        logl = g.full_traversal()
-    G.lib().plk_synchronize(g.context())
-    el = time.perf_counter() - t0
-    k_ms, k_n = g.timing(0)
-    g.set_timing(False)
-    ups = upd_per_step * args.steps / el
-    k_avg_s = k_ms / max(k_n, 1) * 1e-3
-    bytes_per_update = 2 * p.ncat * p.n * 8
-    achieved = upd_per_step * bytes_per_update / k_avg_s / 1e9
-    log(f"fp64: {el / args.steps * 1e3:.3f} ms/step, kernel {k_avg_s * 1e3:.3f} ms, {ups / 1e9:.3f} G upd/s, "
-        f"{achieved:.0f} GB/s algorithmic")
+    def run_form(precision, label):
+        g = G.Tree(p, precision=precision, scaling=G.NORM)
+        for _ in range(args.warmup):
+            g.full_traversal()
+        g.set_timing(True)
+        G.lib().plk_synchronize(g.context())
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            logl = g.full_traversal()
+        G.lib().plk_synchronize(g.context())
+        el = time.perf_counter() - t0
+        k_ms, k_n = g.timing(0)
+        g.set_timing(False)
+        g.close()
+        ups = upd_per_step * args.steps / el
+        k_avg_s = k_ms / max(k_n, 1) * 1e-3
+        w = 8 if precision == G.FP64 else 4
+        bpu = 2 * p.ncat * p.n * w
+        achieved = upd_per_step * bpu / k_avg_s / 1e9
+        log(f"{label}: {el / args.steps * 1e3:.3f} ms/step, kernel {k_avg_s * 1e3:.3f} ms, {ups / 1e9:.3f} G upd/s, "
+            f"{achieved:.0f} GB/s algorithmic, logL {logl:.6f}")
+        return {"ups": ups, "ms_per_step": el / args.steps * 1e3, "kernel_avg_ms": k_avg_s * 1e3,
+                "achieved": achieved, "bytes_per_update": bpu, "logl": logl}
+
+    f64 = run_form(G.FP64, "fp64")
+    f32 = run_form(G.FP32, "fp32")
 
     cpu = None
-    rel_err = None
+    err64 = err32 = None
     if not args.no_cpu:
         ref, cpu = cpu_baseline(p)
-        rel_err = abs(logl - ref) / abs(ref)
-        log(f"cpu: {cpu['value'] / 1e9:.4f} G upd/s on {cpu['cores']} threads; |dlogL|/|logL| = {rel_err:.2e}")
+        err64 = abs(f64["logl"] - ref) / abs(ref)
+        err32 = abs(f32["logl"] - ref) / abs(ref)
+        log(f"cpu: {cpu['value'] / 1e9:.4f} G upd/s on {cpu['cores']} threads; |dlogL|/|logL| fp64 {err64:.2e}, "
+            f"fp32 {err32:.2e}")
 
     out = {
         "metric": METRIC,
-        "value": ups,
+        "value": f32["ups"],
         "unit": "updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": el / args.steps * 1e3,
+        "ms_per_step": f32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic (seeded simulation, resident in HBM)",
         "config": {
             "workload": "protein20_G4_100taxa_50000sites_full_traversal",
             "taxa": p.ntaxa, "sites": p.nsites, "patterns": p.nptn, "states": p.n, "ncat": p.ncat,
-            "form": "fp64", "updates_per_step": upd_per_step, "logl": logl, "logl_rel_err_vs_oracle": rel_err,
+            "form": "fp32 partials on matrix cores, fp64 P(t)/exponents/logL reduction",
+            "updates_per_step": upd_per_step,
+            "logl_fp32": f32["logl"], "logl_rel_err_fp32_vs_oracle": err32,
+            "fp64": {"value": f64["ups"], "ms_per_step": f64["ms_per_step"], "logl": f64["logl"],
+                     "logl_rel_err_vs_oracle": err64, "kernel": "k_traverse_f64<20,MODE_LOGL>",
+                     "kernel_avg_ms": f64["kernel_avg_ms"], "achieved_GBps": f64["achieved"],
+                     "roofline_frac": f64["achieved"] / HBM_PEAK_GBS},
+            "fp32_over_fp64": f32["ups"] / f64["ups"],
+            "gpu_fp32_over_cpu": (f32["ups"] / cpu["value"]) if cpu else None,
         },
         "roofline": {
-            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "k_traverse_f64<20,MODE_LOGL>", "kernel_avg_ms": k_avg_s * 1e3,
-            "alg_bytes_per_update": bytes_per_update,
+            "bound": "hbm", "achieved": f32["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": f32["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "k_traverse_f32_mfma<20,MODE_LOGL>", "kernel_avg_ms": f32["kernel_avg_ms"],
+            "alg_bytes_per_update": f32["bytes_per_update"],
         },
         "cpu_baseline": cpu,
     }

@@ -67,6 +67,7 @@ _SIGS = {
     "plk_get_timing": (_int, [_P, _int, _dp, _i64p]),
     "plk_updates": (ctypes.c_int64, [_P]),
     "plk_synchronize": (_int, [_P]),
+    "plk_get_exponents": (_int, [_P, _int, _ip]),
     # model / synthetic data
     "plk_state_unknown": (_int, [_int, _int]),
     "plk_rate_matrix": (_int, [_int, _dp, _dp, _dp]),

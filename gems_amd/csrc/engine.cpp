@@ -32,8 +32,13 @@ static int64_t round_up(int64_t v, int64_t g) { return (v + g - 1) / g * g; }
 
 static bool supported_states(int n) { return n == 4 || n == 20 || n == 61; }
 
-// per-op matrix block: echild for both children + leaf tables for both children
-static int64_t mat_stride_of(int n, int ncat, int su) {
+// per-op matrix block (in reals): transition data for both children + leaf tables for
+// both children.  FP64: echild[c][x][i]; FP32: matrix-core fragments of P_c.
+static int64_t mat_stride_of(int n, int ncat, int su, int prec) {
+    if (prec == PLK_FP32) {
+        int64_t fr = (int64_t)f32_xblocks(n) * f32_ksteps(n) * 64;
+        return round_up(2LL * ncat * fr + 2LL * (su + 1) * n * ncat, 64);
+    }
     return round_up(2LL * ncat * n * n + 2LL * (su + 1) * n * ncat, 32);
 }
 
@@ -78,7 +83,7 @@ static KParams make_params(Ctx &c) {
     p.part = c.d_part;
     p.slot_stride = (int64_t)c.nptn_pad * c.block;
     p.scale = c.d_scale;
-    p.scale_stride = (int64_t)c.nptn_pad * (c.safe ? c.ncat : 1);
+    p.scale_stride = (int64_t)c.nptn_pad * ((c.safe && c.prec == PLK_FP64) ? c.ncat : 1);
     p.states = c.d_states;
     p.freq = c.d_freq;
     p.invar = c.d_invar;
@@ -90,6 +95,9 @@ static KParams make_params(Ctx &c) {
     p.props = c.d_props;
     p.tip_lh = c.d_tip;
     p.evec = c.d_evec;
+    p.pi = c.d_pi;
+    p.piU = c.d_piU;
+    p.tipvec = c.d_tipvec;
     p.ops = c.d_ops;
     p.nops = 0;
     p.ptn_lh = c.d_ptn_lh;
@@ -121,27 +129,46 @@ static void timing_collect(Ctx &c, const bool used[3]) {
 }
 
 // Upload the queued ops, run the P(t) kernel, then one traversal launch in `mode`.
+// The FP32 form also needs P(t) of the branch itself (an extra job after the ops) for
+// the logL stage, and computes theta in a separate FP64 kernel after the traversal.
 static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     const int nops = (int)c.pending.size();
-    int rc = ensure_op_capacity(c, std::max(nops, 1));
+    const bool f32 = c.prec == PLK_FP32;
+    if (f32 && mode == MODE_LOGL) c.jobs.push_back(PJob{nops, 0, 0, 0, p.br_len});
+    int rc = ensure_op_capacity(c, nops + 1);
     if (rc) return rc;
     p.ops = c.d_ops;
     p.mats = c.d_mats;
     p.nops = nops;
-    if (nops > 0) {
-        size_t ob = sizeof(DevOp) * nops, jb = sizeof(PJob) * c.jobs.size();
+    const int njobs = (int)c.jobs.size();
+    if (njobs > 0) {
+        size_t ob = sizeof(DevOp) * nops, jb = sizeof(PJob) * njobs;
         rc = alloc_stage(c, ob + jb);
         if (rc) return rc;
-        std::memcpy(c.h_stage, c.pending.data(), ob);
+        if (nops > 0) std::memcpy(c.h_stage, c.pending.data(), ob);
         std::memcpy((char *)c.h_stage + ob, c.jobs.data(), jb);
-        HIP_TRY(hipMemcpyAsync(c.d_ops, c.h_stage, ob, hipMemcpyHostToDevice, c.stream));
+        if (nops > 0) HIP_TRY(hipMemcpyAsync(c.d_ops, c.h_stage, ob, hipMemcpyHostToDevice, c.stream));
         HIP_TRY(hipMemcpyAsync(c.d_jobs, (char *)c.h_stage + ob, jb, hipMemcpyHostToDevice, c.stream));
         timing_begin(c, 1);
-        HIP_TRY(f64_pmatrix(p, c.d_jobs, (int)c.jobs.size(), c.stream));
+        HIP_TRY(f32 ? f32_pmatrix(p, c.d_jobs, njobs, c.stream) : f64_pmatrix(p, c.d_jobs, njobs, c.stream));
         timing_end(c, 1);
         used[1] = true;
     }
-    if (nops > 0 || mode != MODE_PARTIALS) {
+    if (f32) {
+        const int tmode = mode == MODE_THETA ? MODE_PARTIALS : mode;
+        if (nops > 0 || tmode != MODE_PARTIALS) {
+            timing_begin(c, 0);
+            HIP_TRY(f32_traverse(p, tmode, c.stream));
+            timing_end(c, 0);
+            used[0] = true;
+        }
+        if (mode == MODE_THETA) {
+            timing_begin(c, 2);
+            HIP_TRY(f32_theta(p, c.stream));
+            timing_end(c, 2);
+            used[2] = true;
+        }
+    } else if (nops > 0 || mode != MODE_PARTIALS) {
         timing_begin(c, 0);
         HIP_TRY(f64_traverse(p, mode, c.stream));
         timing_end(c, 0);
@@ -180,9 +207,9 @@ extern "C" int64_t plk_memory_required(int nstates, int ncat, int ntaxa, int npt
     int64_t su = nstates == 4 ? 18 : nstates == 20 ? 23 : nstates;
     int64_t part = (int64_t)nslots * pad * block * rs;
     int64_t scale = (int64_t)nslots * pad * (precision == PLK_FP64 ? (scaling ? ncat : 1) * 2 : 4);
-    int64_t theta = pad * block * rs;
+    int64_t theta = pad * block * 8;
     int64_t states = (int64_t)ntaxa * pad;
-    int64_t mats = (int64_t)(ntaxa) * mat_stride_of(nstates, ncat, (int)su) * rs;
+    int64_t mats = (int64_t)(ntaxa + 1) * mat_stride_of(nstates, ncat, (int)su, precision) * rs;
     int64_t misc = pad * 8 * 4 + (pad / 64 + 1) * 16;
     return part + scale + theta + states + mats + misc;
 }
@@ -195,7 +222,7 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     if (!supported_states(nstates)) return fail(PLK_ERR_ARG, "plk_create: supported state counts are 4, 20, 61");
     if (ncat < 1 || ncat > PLK_GPU_MAX_CATS) return fail(PLK_ERR_ARG, "plk_create: ncat must be in [1, 8]");
     if (ntaxa < 3 || nptn < 1 || nslots < 1) return fail(PLK_ERR_ARG, "plk_create: need ntaxa >= 3, nptn >= 1, nslots >= 1");
-    if (precision != PLK_FP64) return fail(PLK_ERR_ARG, "plk_create: precision form not available");
+    if (precision != PLK_FP64 && precision != PLK_FP32) return fail(PLK_ERR_ARG, "plk_create: bad precision form");
     if (scaling != PLK_SCALE_NORM && scaling != PLK_SCALE_SAFE) return fail(PLK_ERR_ARG, "plk_create: bad scaling");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
@@ -215,10 +242,12 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     c->prec = precision;
     c->safe = scaling;
     c->su = nstates == 4 ? 18 : nstates == 20 ? 23 : nstates;
-    c->mat_stride = mat_stride_of(nstates, ncat, c->su);
+    c->mat_stride = mat_stride_of(nstates, ncat, c->su, precision);
     const size_t rs = c->real_size();
     const int64_t slot = (int64_t)c->nptn_pad * c->block;
-    const int64_t scale_slot = (int64_t)c->nptn_pad * (scaling ? ncat : 1);
+    // FP64: uint16 scale_num per pattern (NORM) or per pattern and category (SAFE);
+    // FP32: one int32 power-of-two exponent per pattern.
+    const int64_t scale_slot = precision == PLK_FP32 ? (int64_t)c->nptn_pad * 2 : (int64_t)c->nptn_pad * (scaling ? ncat : 1);
     auto cleanup = [&](int rc) {
         plk_destroy(c);
         return rc;
@@ -236,14 +265,14 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMalloc(&c->d_states, (size_t)ntaxa * c->nptn_pad));
     CREATE_TRY(hipMalloc(&c->d_freq, sizeof(double) * c->nptn_pad));
     CREATE_TRY(hipMalloc(&c->d_invar, sizeof(double) * c->nptn_pad));
-    CREATE_TRY(hipMalloc(&c->d_theta, rs * slot));
+    CREATE_TRY(hipMalloc(&c->d_theta, sizeof(double) * slot));
     CREATE_TRY(hipMalloc(&c->d_buf_scale, sizeof(double) * c->nptn_pad));
     CREATE_TRY(hipMalloc(&c->d_ptn_lh, sizeof(double) * c->nptn_pad));
     CREATE_TRY(hipMalloc(&c->d_tile_sums, sizeof(double) * 2 * c->ntiles));
     CREATE_TRY(hipMalloc(&c->d_result, sizeof(double) * 2));
     CREATE_TRY(hipMalloc(&c->d_counter, sizeof(unsigned)));
     const int n = nstates;
-    const size_t model_doubles = n + 2 * n * n + 2 * ncat + (size_t)(c->su + 1) * n;
+    const size_t model_doubles = n + 2 * n * n + 2 * ncat + 2 * (size_t)(c->su + 1) * n + n + n * n;
     CREATE_TRY(hipMalloc(&c->d_model, sizeof(double) * model_doubles));
     c->d_eval = c->d_model;
     c->d_evec = c->d_eval + n;
@@ -251,6 +280,9 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     c->d_rates = c->d_inv_evec + n * n;
     c->d_props = c->d_rates + ncat;
     c->d_tip = c->d_props + ncat;
+    c->d_tipvec = c->d_tip + (size_t)(c->su + 1) * n;
+    c->d_pi = c->d_tipvec + (size_t)(c->su + 1) * n;
+    c->d_piU = c->d_pi + n;
     CREATE_TRY(hipHostMalloc((void **)&c->h_result, sizeof(double) * 2, hipHostMallocDefault));
     for (int i = 0; i < 6; i++) CREATE_TRY(hipEventCreate(&c->ev[i]));
     // zero-initialised state: counters, scale counts and partials of unused slots
@@ -259,7 +291,7 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMemsetAsync(c->d_freq, 0, sizeof(double) * c->nptn_pad, c->stream));
     CREATE_TRY(hipMemsetAsync(c->d_invar, 0, sizeof(double) * c->nptn_pad, c->stream));
     CREATE_TRY(hipStreamSynchronize(c->stream));
-    int rc = ensure_op_capacity(*c, std::max(16, ntaxa));
+    int rc = ensure_op_capacity(*c, std::max(16, ntaxa + 1));
     if (rc) return cleanup(rc);
 #undef CREATE_TRY
     *out = c;
@@ -322,6 +354,28 @@ extern "C" int plk_set_model(plk_ctx *c, const double *eval, const double *evec,
     m.insert(m.end(), rates, rates + ncat);
     m.insert(m.end(), props, props + ncat);
     m.insert(m.end(), tip_lh, tip_lh + (size_t)(c->su + 1) * n);
+    // FP32-form extras.  Real tip vectors U * tip_lh; pi is the eigenvector row of the
+    // zero eigenvalue of U^-1 (U^-1 = V^T Pi^1/2 with V[:,0] = Pi^1/2), normalised;
+    // piU[i][x] = pi_x U[x][i] maps a real-space partial to eigen space.
+    std::vector<double> tipvec((size_t)(c->su + 1) * n, 0.0), pi(n, 0.0), piU((size_t)n * n, 0.0);
+    for (int s = 0; s <= c->su; s++)
+        for (int x = 0; x < n; x++) {
+            double v = 0.0;
+            for (int i = 0; i < n; i++) v += evec[x * n + i] * tip_lh[s * n + i];
+            tipvec[(size_t)s * n + x] = v;
+        }
+    int i0 = 0;
+    for (int i = 1; i < n; i++)
+        if (std::fabs(eval[i]) < std::fabs(eval[i0])) i0 = i;
+    double psum = 0.0;
+    for (int x = 0; x < n; x++) psum += inv_evec[i0 * n + x];
+    if (!(std::fabs(psum) > 0.0)) return fail(PLK_ERR_ARG, "plk_set_model: no stationary eigenvector");
+    for (int x = 0; x < n; x++) pi[x] = inv_evec[i0 * n + x] / psum;
+    for (int i = 0; i < n; i++)
+        for (int x = 0; x < n; x++) piU[(size_t)i * n + x] = pi[x] * evec[x * n + i];
+    m.insert(m.end(), tipvec.begin(), tipvec.end());
+    m.insert(m.end(), pi.begin(), pi.end());
+    m.insert(m.end(), piU.begin(), piU.end());
     for (double v : m)
         if (!std::isfinite(v)) return fail(PLK_ERR_ARG, "plk_set_model: non-finite model value");
     HIP_TRY(hipMemcpyAsync(c->d_model, m.data(), sizeof(double) * m.size(), hipMemcpyHostToDevice, c->stream));
@@ -505,11 +559,18 @@ extern "C" int plk_get_partial(plk_ctx *c, int slot, double *partial, uint16_t *
     const int64_t slot_len = (int64_t)c->nptn_pad * c->block;
     if (partial) {
         std::vector<double> raw(slot_len);
-        HIP_TRY(hipMemcpy(raw.data(), (char *)c->d_part + 8 * slot_len * slot, 8 * slot_len, hipMemcpyDeviceToHost));
+        if (c->prec == PLK_FP32) {
+            std::vector<float> rf(slot_len);
+            HIP_TRY(hipMemcpy(rf.data(), (char *)c->d_part + 4 * slot_len * slot, 4 * slot_len, hipMemcpyDeviceToHost));
+            std::copy(rf.begin(), rf.end(), raw.begin());
+        } else {
+            HIP_TRY(hipMemcpy(raw.data(), (char *)c->d_part + 8 * slot_len * slot, 8 * slot_len, hipMemcpyDeviceToHost));
+        }
         for (int p = 0; p < c->nptn; p++)
             for (int i = 0; i < c->block; i++)
                 partial[(int64_t)p * c->block + i] = raw[((int64_t)(p / 64) * c->block + i) * 64 + p % 64];
     }
+    if (scale && c->prec == PLK_FP32) return fail(PLK_ERR_ARG, "plk_get_partial: the FP32 form keeps exponents (plk_get_exponents)");
     if (scale) {
         const int64_t ss = (int64_t)c->nptn_pad * (c->safe ? c->ncat : 1);
         std::vector<uint16_t> raw(ss);
@@ -541,5 +602,16 @@ extern "C" int64_t plk_updates(plk_ctx *c) { return c ? c->updates : 0; }
 extern "C" int plk_synchronize(plk_ctx *c) {
     if (!c) return fail(PLK_ERR_ARG, "null context");
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return PLK_OK;
+}
+
+extern "C" int plk_get_exponents(plk_ctx *c, int slot, int32_t *exponents) {
+    clear_error();
+    if (!c || slot < 0 || slot >= c->nslots || !exponents) return fail(PLK_ERR_ARG, "plk_get_exponents: bad arguments");
+    if (c->prec != PLK_FP32) return fail(PLK_ERR_ARG, "plk_get_exponents: only the FP32 form keeps exponents");
+    int rc = plk_flush(c);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(exponents, (char *)c->d_scale + (int64_t)4 * c->nptn_pad * slot, sizeof(int32_t) * c->nptn,
+                      hipMemcpyDeviceToHost));
     return PLK_OK;
 }

@@ -12,6 +12,10 @@
 // Largest category count the GPU kernels hold per pattern in LDS.
 #define PLK_GPU_MAX_CATS 8
 
+// FP32-form matrix-core tiling: 16-row x blocks and 4-deep k steps of a state count.
+constexpr int f32_xblocks(int n) { return (n + 15) / 16; }
+constexpr int f32_ksteps(int n) { return (n + 3) / 4; }
+
 namespace plk {
 
 // Device-side op: one directional partial update.  kind: 0 tip-tip, 1 tip-internal
@@ -47,6 +51,7 @@ struct KParams {
     const void *mats;         // per-op matrices
     int64_t mat_stride;       // reals per op
     const double *inv_evec, *eval, *rates, *props, *tip_lh, *evec;
+    const double *pi, *piU, *tipvec;  // FP32 form: frequencies, pi_x U[x][i] as [i][x], real tip vectors
     const DevOp *ops;
     int nops;
     int br_a, br_b;           // branch stage: node refs (br_a may be a tip), br_b a slot
@@ -75,7 +80,7 @@ struct Ctx {
     int mats_cap = 0;
     double *d_model = nullptr;  // eval | evec | inv_evec | rates | props | tip_lh
     double *d_eval = nullptr, *d_evec = nullptr, *d_inv_evec = nullptr, *d_rates = nullptr, *d_props = nullptr,
-           *d_tip = nullptr;
+           *d_tip = nullptr, *d_pi = nullptr, *d_piU = nullptr, *d_tipvec = nullptr;
     DevOp *d_ops = nullptr;
     PJob *d_jobs = nullptr;
     int ops_cap = 0;
@@ -108,4 +113,7 @@ namespace plk {
 hipError_t f64_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
 hipError_t f64_traverse(const KParams &p, int mode, hipStream_t s);
 hipError_t f64_theta_eval(const KParams &p, int deriv, hipStream_t s);
+hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
+hipError_t f32_traverse(const KParams &p, int mode, hipStream_t s);
+hipError_t f32_theta(const KParams &p, hipStream_t s);
 }  // namespace plk

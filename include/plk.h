@@ -123,6 +123,10 @@ int plk_loglik_from_theta(plk_ctx *ctx, double t, double *logl);
  * nptn*ncat uint16 [ptn][c]).  Either pointer may be NULL. */
 int plk_get_partial(plk_ctx *ctx, int slot, double *partial, uint16_t *scale);
 
+/* FP32 form only: the per-pattern power-of-two exponents of one slot (true partial =
+ * stored value * 2^exponent), nptn int32. */
+int plk_get_exponents(plk_ctx *ctx, int slot, int32_t *exponents);
+
 /* Kernel timing with HIP events recorded on the context's stream around every launch
  * of the partial-traversal kernel.  kind: 0 traversal, 1 P(t), 2 derivative. */
 int plk_set_timing(plk_ctx *ctx, int enable);
