@@ -43,6 +43,18 @@ _int = ctypes.c_int
 _dbl = ctypes.c_double
 
 
+class plk_request(ctypes.Structure):
+    _fields_ = [("nstates", ctypes.c_int), ("ncat", ctypes.c_int), ("ntaxa", ctypes.c_int), ("nptn", ctypes.c_int),
+                ("precision", ctypes.c_int), ("scaling", ctypes.c_int), ("state_unknown", ctypes.c_int),
+                ("eval", _dp), ("evec", _dp), ("inv_evec", _dp), ("rates", _dp), ("props", _dp), ("tip_lh", _dp),
+                ("states", _u8p), ("freq", _dp), ("invar", _dp), ("nei", _ip), ("len", _dp),
+                ("kind", ctypes.c_int), ("opt_a", ctypes.c_int), ("opt_b", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("tol", ctypes.c_double),
+                ("logl_before", ctypes.c_double), ("logl_after", ctypes.c_double), ("opt_len", ctypes.c_double),
+                ("status", ctypes.c_int), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("footprint", ctypes.c_int64)]
+
+
 class plk_op(ctypes.Structure):
     _fields_ = [("out_slot", ctypes.c_int32), ("child", ctypes.c_int32 * 2), ("reserved", ctypes.c_int32),
                 ("len", ctypes.c_double * 2)]
@@ -98,6 +110,23 @@ _SIGS = {
     "plk_tree_optimize_all_branches": (_int, [_P, _int, _dbl, _dbl, _dbl, _int, _dp]),
     "plk_tree_full_traversal": (_int, [_P, _dp]),
     "plk_tree_counters": (_int, [_P, _i64p]),
+    "plk_tree_queue_branch": (_int, [_P, _int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp]),
+    "plk_sim_create_on_tree": (_int, [ctypes.POINTER(_P), _int, _int, _ip, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
+    # batched partitions
+    "plk_batch_create": (_int, [ctypes.POINTER(_P), ctypes.POINTER(_P), _int]),
+    "plk_batch_destroy": (_int, [_P]),
+    "plk_batch_set_launch": (_int, [_P, _int, _int]),
+    "plk_batch_loglik": (_int, [_P, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp, _dp, _dp]),
+    "plk_batch_autotune": (_int, [_P, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp, _int, ctypes.POINTER(_int), _dp]),
+    "plk_batch_timing": (_int, [_P, _dp, _i64p, _i64p]),
+    "plk_batch_items": (_int, [_P]),
+    # serving
+    "plk_server_create": (_int, [ctypes.POINTER(_P), _int, ctypes.c_int64]),
+    "plk_server_submit": (_int, [_P, ctypes.c_void_p]),
+    "plk_server_drain": (_int, [_P]),
+    "plk_server_now": (ctypes.c_double, [_P]),
+    "plk_server_stats": (_int, [_P, _dp]),
+    "plk_server_destroy": (_int, [_P]),
 }
 
 
@@ -142,10 +171,15 @@ def device_count():
 class Problem:
     """Seeded synthetic problem generated by the library (plk_sim_create)."""
 
-    def __init__(self, seq_type, ntaxa, nsites, mean_len=0.1, alpha=0.5, ncat=4, seed=1):
+    def __init__(self, seq_type, ntaxa, nsites, mean_len=0.1, alpha=0.5, ncat=4, seed=1, topology=None):
         L = lib()
         h = _P()
-        _check(L.plk_sim_create(ctypes.byref(h), seq_type, ntaxa, nsites, mean_len, alpha, ncat, seed))
+        if topology is None:
+            _check(L.plk_sim_create(ctypes.byref(h), seq_type, ntaxa, nsites, mean_len, alpha, ncat, seed))
+        else:
+            nei = np.ascontiguousarray(topology, dtype=np.int32)
+            _check(L.plk_sim_create_on_tree(ctypes.byref(h), seq_type, ntaxa, nei.ctypes.data_as(_ip), nsites,
+                                            mean_len, alpha, ncat, seed))
         self._h = h
         dims = np.zeros(7, dtype=np.int32)
         _check(L.plk_sim_dims(h, dims.ctypes.data_as(_ip)))
@@ -266,3 +300,132 @@ class Tree:
         ms, n = ctypes.c_double(), ctypes.c_int64()
         _check(lib().plk_get_timing(self.context(), kind, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def queue_branch(self, a, b, clear=False):
+        ra, rb, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(lib().plk_tree_queue_branch(self.h, a, b, int(clear), ctypes.byref(ra), ctypes.byref(rb),
+                                           ctypes.byref(t)))
+        return ra.value, rb.value, t.value
+
+
+class Batch:
+    """Batched unlinked partitions (plk_batch_*): one launch per traversal for all trees."""
+
+    def __init__(self, trees):
+        self.trees = list(trees)
+        arr = (_P * len(self.trees))(*[t.context() for t in self.trees])
+        h = _P()
+        _check(lib().plk_batch_create(ctypes.byref(h), arr, len(self.trees)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().plk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_launch(self, grid=0, use_graph=True):
+        _check(lib().plk_batch_set_launch(self.h, grid, int(use_graph)))
+
+    def _queue(self, branches, clear):
+        n = len(self.trees)
+        a, b, t = (_int * n)(), (_int * n)(), np.zeros(n)
+        for i, (tr, (u, v)) in enumerate(zip(self.trees, branches)):
+            a[i], b[i], t[i] = tr.queue_branch(u, v, clear)
+        return a, b, t
+
+    def loglik(self, branches, clear=False):
+        a, b, t = self._queue(branches, clear)
+        out = np.zeros(len(self.trees))
+        tot = ctypes.c_double()
+        _check(lib().plk_batch_loglik(self.h, a, b, _d(t), _d(out), ctypes.byref(tot)))
+        return out, tot.value
+
+    def autotune(self, branches, reps=3):
+        a, b, t = self._queue(branches, True)
+        g, ms = _int(), ctypes.c_double()
+        _check(lib().plk_batch_autotune(self.h, a, b, _d(t), reps, ctypes.byref(g), ctypes.byref(ms)))
+        # the tuned traversal stays queued: evaluate it once so the trees stay consistent
+        out = np.zeros(len(self.trees))
+        tot = ctypes.c_double()
+        _check(lib().plk_batch_loglik(self.h, a, b, _d(t), _d(out), ctypes.byref(tot)))
+        return g.value, ms.value
+
+    def timing(self):
+        ms, n, u = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().plk_batch_timing(self.h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(u)))
+        return ms.value, n.value, u.value
+
+    def items(self):
+        return lib().plk_batch_items(self.h)
+
+
+REQ_LOGL, REQ_OPT_BRANCH, REQ_OPT_ALL = 0, 1, 2
+
+
+class Request:
+    """One serving request built from a Problem; keeps its arrays alive."""
+
+    def __init__(self, prob, kind=REQ_LOGL, precision=FP64, scaling=NORM, opt_branch=None, iterations=1, tol=1e-3):
+        self.p = prob
+        r = plk_request()
+        keep = []
+
+        def arr(a, dt=np.float64):
+            a = np.ascontiguousarray(a, dtype=dt).ravel()
+            keep.append(a)
+            return a
+
+        r.nstates, r.ncat, r.ntaxa, r.nptn = prob.n, len(prob.rates), prob.ntaxa, prob.nptn
+        r.precision, r.scaling = precision, scaling
+        tip = arr(prob.tip_lh)
+        r.state_unknown = tip.size // prob.n - 1
+        r.eval, r.evec, r.inv_evec = _d(arr(prob.eval)), _d(arr(prob.evec)), _d(arr(prob.inv_evec))
+        r.rates, r.props, r.tip_lh = _d(arr(prob.rates)), _d(arr(prob.props)), _d(tip)
+        r.states = arr(prob.states, np.uint8).ctypes.data_as(_u8p)
+        r.freq = _d(arr(prob.freq))
+        r.invar = None
+        r.nei = arr(prob.nei, np.int32).ctypes.data_as(_ip)
+        r.len = _d(arr(prob.blen))
+        r.kind = kind
+        a, b = opt_branch if opt_branch else (0, int(prob.nei[0, 0]))
+        r.opt_a, r.opt_b, r.iterations, r.tol = a, b, iterations, tol
+        self.r = r
+        self._keep = keep
+
+    def result(self):
+        r = self.r
+        return {"status": r.status, "logl_before": r.logl_before, "logl_after": r.logl_after, "opt_len": r.opt_len,
+                "latency": r.t_done - r.t_submit, "queue": r.t_admit - r.t_submit, "footprint": r.footprint}
+
+
+class Server:
+    def __init__(self, nworkers=4, hbm_budget=200 << 30):
+        h = _P()
+        _check(lib().plk_server_create(ctypes.byref(h), nworkers, hbm_budget))
+        self.h = h
+
+    def submit(self, req):
+        _check(lib().plk_server_submit(self.h, ctypes.addressof(req.r)))
+
+    def drain(self):
+        _check(lib().plk_server_drain(self.h))
+
+    def now(self):
+        return lib().plk_server_now(self.h)
+
+    def stats(self):
+        st = np.zeros(8)
+        _check(lib().plk_server_stats(self.h, _d(st)))
+        keys = ["completed", "failed", "p50_s", "p99_s", "mean_s", "makespan_s", "peak_hbm_bytes", "peak_concurrent"]
+        return dict(zip(keys, st.tolist()))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().plk_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,94 @@
+// Device helpers shared by the FP64, FP32 and batched likelihood kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "engine_common.h"
+
+namespace plk {
+namespace dev {
+
+constexpr double kST = 8.636168555094444625386e-78;  // 2^-256 (tree/phylotree.h:74-80)
+constexpr double kTwo256 = 0x1p256;
+constexpr double kLogST = -177.4456782233459932741;
+constexpr double kLn2 = 0.69314718055994530941723212145818;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Read-only views in the constant address space: wave-uniform reads through them
+// compile to scalar loads whose results feed the FMAs as SGPR operands (the data is
+// written by an earlier launch only).
+typedef const __attribute__((address_space(4))) double *cdptr;
+typedef const __attribute__((address_space(4))) float *cfptr;
+__device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
+__device__ __forceinline__ cfptr as_constf(const float *p) { return (cfptr)p; }
+
+__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
+__device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
+
+// Deterministic wavefront sum (fixed shuffle tree); result valid in lane 0.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Compile-time loop: f(I) for I = B..E-1, fully unrolled.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+// x-loop over output states: unrolled for small state counts, a plain loop for the
+// 61-state codon model.
+template <int N, typename F>
+__device__ __forceinline__ void for_states(F &&f) {
+    if constexpr (N <= 20) {
+        static_for<0, N>([&](auto X) { f(decltype(X)::value); });
+    } else {
+#pragma unroll 1
+        for (int x = 0; x < N; x++) f(x);
+    }
+}
+
+// Per-tile values -> fixed-order total over the ntiles tiles of one alignment.  The
+// last workgroup to finish a tile of this alignment sums the tile values in index
+// order, so the result does not depend on scheduling (or on other alignments sharing
+// the launch); it then re-arms the counter for the next launch (no memset, graph-safe).
+__device__ inline void finish_tiles(const KParams &P, double v0, double v1, int tile, int ntiles) {
+    __shared__ int s_last;
+    if (threadIdx.x == 0) {
+        P.tile_sums[2 * tile] = v0;
+        P.tile_sums[2 * tile + 1] = v1;
+        __threadfence();
+        unsigned prev = atomicAdd(P.counter, 1u);
+        s_last = (prev == (unsigned)ntiles - 1);
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        if (threadIdx.x < 64) {
+            double a0 = 0.0, a1 = 0.0;
+            for (int i = threadIdx.x; i < ntiles; i += 64) {
+                a0 += __hip_atomic_load(&P.tile_sums[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                a1 += __hip_atomic_load(&P.tile_sums[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            if (threadIdx.x == 0) {
+                P.result[0] = a0;
+                P.result[1] = a1;
+                __hip_atomic_store(P.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace dev
+}  // namespace plk

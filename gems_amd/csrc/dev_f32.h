@@ -1,0 +1,489 @@
+// FP32-form likelihood kernel bodies for gfx950.
+//
+// Partials are real-space conditional likelihoods in FP32 with one int32 power-of-two
+// exponent per pattern and slot: true value = stored * 2^e.  Every op renormalises the
+// pattern so that its largest entry lies in [0.5, 1), so FP32 never underflows for
+// realistic trees.  Transition matrices P_c(t) = U exp(Lambda r_c t) U^-1, the exponent
+// sums, theta, the pattern-weighted logL and the derivative sums stay FP64.
+//
+// 20- and 61-state models run the pattern x state by state x state contractions on
+// the matrix cores (v_mfma_f32_16x16x4f32): for one category, D[x][p] = sum_y P[x][y]
+// L[y][p] with x in 16-row blocks, y in 4-deep k-steps and 16 patterns per block.  The
+// P fragments are stored pre-swizzled so each lane loads its A operand with one
+// coalesced 4-byte load.  DNA uses the vector ALU with scalar-register matrices.
+#pragma once
+
+#include "dev_common.h"
+
+namespace plk {
+namespace dev {
+
+// Renormalisation exponent of a pattern maximum: m * 2^-e in [0.5, 1).
+__device__ __forceinline__ int norm_exponent(float m) {
+    int e = 0;
+    if (m > 0.0f) (void)frexpf(m, &e);
+    return e;
+}
+
+// ---------------------------------------------------------------------------------------
+// P(t) in FP32 form: P_c[x][y] (FP64 math) written as MFMA A fragments
+// frag[c][xb][ks][lane] = P_c[xb*16 + lane%16][ks*4 + lane/16] (zero padded), or plainly
+// [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][x] = (P_c tip_s)[x].
+template <int N>
+__device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    const int ncat = P.ncat, su = P.su;
+    float *M = (float *)P.mats + (int64_t)j.op * P.mat_stride;
+    float *F = M + (int64_t)j.side * ncat * FR;
+    float *T = M + (int64_t)2 * ncat * FR + (int64_t)j.side * (su + 1) * ncat * N;
+    __shared__ double ex[N];
+    __shared__ double Pm[N * N];
+    for (int c = 0; c < ncat; c++) {
+        for (int i = threadIdx.x; i < N; i += blockDim.x) ex[i] = exp(P.eval[i] * (P.rates[c] * j.len));
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
+            const int x = idx / N, y = idx % N;
+            double v = 0.0;
+            for (int i = 0; i < N; i++) v = fma(P.evec[x * N + i] * ex[i], P.inv_evec[i * N + y], v);
+            Pm[idx] = v;
+        }
+        __syncthreads();
+        if (N == 4) {
+            for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) F[(int64_t)c * FR + idx] = (float)Pm[idx];
+        } else {
+            for (int idx = threadIdx.x; idx < FR; idx += blockDim.x) {
+                const int xb = idx / (KS * 64), ks = (idx / 64) % KS, l = idx % 64;
+                const int x = xb * 16 + (l & 15), y = ks * 4 + (l >> 4);
+                F[(int64_t)c * FR + idx] = (x < N && y < N) ? (float)Pm[x * N + y] : 0.0f;
+            }
+        }
+        if (j.leaf) {
+            for (int idx = threadIdx.x; idx < su * N; idx += blockDim.x) {
+                const int s = idx / N, x = idx % N;
+                double v = 0.0;
+                for (int y = 0; y < N; y++) v = fma(Pm[x * N + y], P.tipvec[s * N + y], v);
+                T[((int64_t)s * ncat + c) * N + x] = (float)v;
+            }
+            for (int x = threadIdx.x; x < N; x += blockDim.x) T[((int64_t)su * ncat + c) * N + x] = 1.0f;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Matrix-core traversal (20 and 61 states).  Workgroup = one 64-pattern tile, one
+// wavefront per category.  Lane l of a wave owns pattern column l%16 of each 16-pattern
+// block and rows 4*(l/16)..+3 of each 16-row x block of the MFMA output.
+template <int N, int MODE>
+__device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) {
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
+    const int ncat = P.ncat;
+    // one category per wave; waves beyond ncat (batched launches) only join barriers
+    const int cw = wave_id();
+    const bool active = cw < ncat;
+    const int c = active ? cw : 0;
+    float *__restrict__ part = (float *)P.part;
+    int *__restrict__ expo = (int *)P.scale;
+    const int64_t cbase = ((int64_t)tile * ncat + c) * N * 64;  // (tile, c, x=0, lane=0)
+    __shared__ float s_max[2][PLK_GPU_MAX_CATS][64];
+    __shared__ double s_cat[PLK_GPU_MAX_CATS][64];
+    __shared__ float s_pi[64];
+    if (MODE == MODE_LOGL && threadIdx.x < N) s_pi[threadIdx.x] = (float)P.pi[threadIdx.x];
+
+    for (int k = 0; k < P.nops; k++) {
+        const DevOp op = P.ops[k];
+        const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
+        const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
+        floatx4 out[4][XB];
+        if (!active) {
+#pragma unroll
+            for (int pb = 0; pb < 4; pb++)
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++) out[pb][xb] = floatx4{0, 0, 0, 0};
+        } else if (op.kind == KIND_TT) {
+            const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
+            const uint8_t *str = P.states + (int64_t)op.right * P.nptn_pad + (int64_t)tile * 64;
+            const float *tl0 = tabs;
+            const float *tr0 = tabs + (int64_t)(P.su + 1) * ncat * N;
+#pragma unroll
+            for (int pb = 0; pb < 4; pb++) {
+                const int sl = stl[pb * 16 + l16], sr = str[pb * 16 + l16];
+                const float *tl = tl0 + ((int64_t)sl * ncat + c) * N;
+                const float *tr = tr0 + ((int64_t)sr * ncat + c) * N;
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int x = xb * 16 + lg * 4 + r;
+                        out[pb][xb][r] = (x < N) ? tl[x] * tr[x] : 0.0f;
+                    }
+            }
+        } else {
+            // right child is internal: D_R = P_R . R on the matrix cores
+            const float *__restrict__ FRm = M + (int64_t)(ncat + c) * FR;
+            float aR[XB][KS];
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) aR[xb][ks] = FRm[(xb * KS + ks) * 64 + lane];
+            const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
+            if (op.kind == KIND_II) {
+                const float *__restrict__ FLm = M + (int64_t)c * FR;
+                float aL[XB][KS];
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int ks = 0; ks < KS; ks++) aL[xb][ks] = FLm[(xb * KS + ks) * 64 + lane];
+                const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase;
+#pragma unroll
+                for (int pb = 0; pb < 4; pb++) {
+                    floatx4 dL[XB], dR[XB];
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++) {
+                        dL[xb] = floatx4{0, 0, 0, 0};
+                        dR[xb] = floatx4{0, 0, 0, 0};
+                    }
+#pragma unroll
+                    for (int ks = 0; ks < KS; ks++) {
+                        const int y = ks * 4 + lg;
+                        const float bl = (y < N) ? L[y * 64 + pb * 16 + l16] : 0.0f;
+                        const float br = (y < N) ? R[y * 64 + pb * 16 + l16] : 0.0f;
+#pragma unroll
+                        for (int xb = 0; xb < XB; xb++) {
+                            dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[xb][ks], bl, dL[xb], 0, 0, 0);
+                            dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], br, dR[xb], 0, 0, 0);
+                        }
+                    }
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++) out[pb][xb] = dL[xb] * dR[xb];
+                }
+            } else {
+                const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
+#pragma unroll
+                for (int pb = 0; pb < 4; pb++) {
+                    floatx4 dR[XB];
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++) dR[xb] = floatx4{0, 0, 0, 0};
+#pragma unroll
+                    for (int ks = 0; ks < KS; ks++) {
+                        const int y = ks * 4 + lg;
+                        const float br = (y < N) ? R[y * 64 + pb * 16 + l16] : 0.0f;
+#pragma unroll
+                        for (int xb = 0; xb < XB; xb++)
+                            dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], br, dR[xb], 0, 0, 0);
+                    }
+                    const int sl = stl[pb * 16 + l16];
+                    const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int x = xb * 16 + lg * 4 + r;
+                            out[pb][xb][r] = (x < N) ? tl[x] * dR[xb][r] : 0.0f;
+                        }
+                }
+            }
+        }
+        // per-pattern maximum over this category, then over all categories (LDS)
+#pragma unroll
+        for (int pb = 0; pb < 4; pb++) {
+            float m = 0.0f;
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) m = fmaxf(m, fabsf(out[pb][xb][r]));
+            m = fmaxf(m, __shfl_xor(m, 16, 64));
+            m = fmaxf(m, __shfl_xor(m, 32, 64));
+            if (lg == 0 && active) s_max[k & 1][c][pb * 16 + l16] = m;
+        }
+        __syncthreads();
+        float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride + cbase;
+#pragma unroll
+        for (int pb = 0; pb < 4; pb++) {
+            const int pl = pb * 16 + l16;
+            float pm = 0.0f;
+            for (int cc = 0; cc < ncat; cc++) pm = fmaxf(pm, s_max[k & 1][cc][pl]);
+            const int e = norm_exponent(pm);
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int x = xb * 16 + lg * 4 + r;
+                    if (x < N && active) outp[x * 64 + pl] = ldexpf(out[pb][xb][r], -e);
+                }
+            if (cw == 0 && lg == 0) {
+                const int64_t p = (int64_t)tile * 64 + pl;
+                int es = e;
+                if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
+                else if (op.kind == KIND_II)
+                    es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
+                expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
+            }
+        }
+        // the next op reads these values from other lanes of this wavefront
+        __syncthreads();
+    }
+    if (MODE == MODE_PARTIALS) return;
+    __syncthreads();  // s_pi visible even when the op list is empty
+
+    // -------- branch logL: lh_c = sum_x pi_x A[x] (P_c(t) B)[x]
+    const float *__restrict__ Mt = (const float *)P.mats + (int64_t)P.nops * P.mat_stride + (int64_t)c * FR;
+    float at[XB][KS];
+#pragma unroll
+    for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) at[xb][ks] = Mt[(xb * KS + ks) * 64 + lane];
+    const bool atip = P.br_a < 0;
+    const float *__restrict__ A = atip ? nullptr : part + (int64_t)P.br_a * P.slot_stride + cbase;
+    const float *__restrict__ B = part + (int64_t)P.br_b * P.slot_stride + cbase;
+    const uint8_t *sta = atip ? P.states + (int64_t)(-P.br_a - 1) * P.nptn_pad + (int64_t)tile * 64 : nullptr;
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) {
+        floatx4 d[XB];
+#pragma unroll
+        for (int xb = 0; xb < XB; xb++) d[xb] = floatx4{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int y = ks * 4 + lg;
+            const float b = (y < N) ? B[y * 64 + pb * 16 + l16] : 0.0f;
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++) d[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(at[xb][ks], b, d[xb], 0, 0, 0);
+        }
+        const double *tv = atip ? P.tipvec + (int64_t)sta[pb * 16 + l16] * N : nullptr;
+        float acc = 0.0f;
+#pragma unroll
+        for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int x = xb * 16 + lg * 4 + r;
+                if (x < N) {
+                    const float a = atip ? (float)tv[x] : A[x * 64 + pb * 16 + l16];
+                    acc = fmaf(s_pi[x] * a, d[xb][r], acc);
+                }
+            }
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        if (lg == 0 && active) s_cat[c][pb * 16 + l16] = (double)acc;
+    }
+    __syncthreads();
+    double r0 = 0.0;
+    if (cw == 0) {
+        const int64_t p = (int64_t)tile * 64 + lane;
+        double lh = 0.0;
+        for (int cc = 0; cc < ncat; cc++) lh += P.props[cc] * s_cat[cc][lane];
+        lh = fabs(lh) + P.invar[p];
+        const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
+        const double l = log(lh) + e * kLn2;
+        if (p < P.nptn) P.ptn_lh[p] = l;
+        r0 = wave_sum(p < P.nptn ? l * P.freq[p] : 0.0);
+    }
+    finish_tiles(P, r0, 0.0, tile, P.ntiles);
+}
+
+// ---------------------------------------------------------------------------------------
+// Vector-ALU traversal for DNA: thread per (pattern, category), matrices in SGPRs.
+template <int N, int MODE>
+__device__ inline void traverse_f32_valu_body(const KParams &P, const int tile) {
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    const int lane = threadIdx.x & 63;
+    const int w = wave_id();
+    const int ncw = nwaves(), ncat = P.ncat;
+    const int64_t p = (int64_t)tile * 64 + lane;
+    float *__restrict__ part = (float *)P.part;
+    int *__restrict__ expo = (int *)P.scale;
+    __shared__ float s_max[2][4][64];
+    __shared__ double s_cat[PLK_GPU_MAX_CATS][64];
+    constexpr int CPW = (PLK_GPU_MAX_CATS + 3) / 4;  // categories per wave at most
+
+    for (int k = 0; k < P.nops; k++) {
+        const DevOp op = P.ops[k];
+        const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
+        const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
+        int sl = 0, sr = 0;
+        if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
+        if (op.kind == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
+        float out[CPW][N];
+        float wmax = 0.0f;
+#pragma unroll
+        for (int ci = 0; ci < CPW; ci++) {
+            const int c = w + ci * ncw;
+            if (c >= ncat) break;
+            const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+            const cfptr PL = as_constf(M + (int64_t)c * FR);
+            const cfptr PR = as_constf(M + (int64_t)(ncat + c) * FR);
+            float l[N], r[N];
+            if (op.kind == KIND_II) {
+                const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + off;
+#pragma unroll
+                for (int y = 0; y < N; y++) l[y] = L[y * 64];
+            }
+            if (op.kind != KIND_TT) {
+                const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + off;
+#pragma unroll
+                for (int y = 0; y < N; y++) r[y] = R[y * 64];
+            }
+            const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+            const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) {
+                float a, b;
+                if (op.kind == KIND_II) {
+                    a = 0.0f;
+#pragma unroll
+                    for (int y = 0; y < N; y++) a = fmaf(PL[x * N + y], l[y], a);
+                } else {
+                    a = tl[x];
+                }
+                if (op.kind == KIND_TT) {
+                    b = tr[x];
+                } else {
+                    b = 0.0f;
+#pragma unroll
+                    for (int y = 0; y < N; y++) b = fmaf(PR[x * N + y], r[y], b);
+                }
+                out[ci][x] = a * b;
+                wmax = fmaxf(wmax, fabsf(out[ci][x]));
+            }
+        }
+        s_max[k & 1][w][lane] = wmax;
+        __syncthreads();
+        float pm = 0.0f;
+        for (int ww = 0; ww < ncw; ww++) pm = fmaxf(pm, s_max[k & 1][ww][lane]);
+        const int e = norm_exponent(pm);
+        float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride;
+#pragma unroll
+        for (int ci = 0; ci < CPW; ci++) {
+            const int c = w + ci * ncw;
+            if (c >= ncat) break;
+            const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+#pragma unroll
+            for (int x = 0; x < N; x++) outp[off + x * 64] = ldexpf(out[ci][x], -e);
+        }
+        if (w == 0) {
+            int es = e;
+            if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
+            else if (op.kind == KIND_II)
+                es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
+            expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
+        }
+    }
+    if (MODE == MODE_PARTIALS) return;
+
+    const float *__restrict__ Mt = (const float *)P.mats + (int64_t)P.nops * P.mat_stride;
+    const bool atip = P.br_a < 0;
+    const int sa = atip ? P.states[(int64_t)(-P.br_a - 1) * P.nptn_pad + p] : 0;
+    for (int c = w; c < ncat; c += ncw) {
+        const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+        const cfptr Pt = as_constf(Mt + (int64_t)c * FR);
+        const float *__restrict__ B = (const float *)P.part + (int64_t)P.br_b * P.slot_stride + off;
+        const float *__restrict__ A = (const float *)P.part + (int64_t)(atip ? 0 : P.br_a) * P.slot_stride + off;
+        float b[N];
+#pragma unroll
+        for (int y = 0; y < N; y++) b[y] = B[y * 64];
+        float acc = 0.0f;
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            float d = 0.0f;
+#pragma unroll
+            for (int y = 0; y < N; y++) d = fmaf(Pt[x * N + y], b[y], d);
+            const float a = atip ? (float)P.tipvec[sa * N + x] : A[x * 64];
+            acc = fmaf((float)P.pi[x] * a, d, acc);
+        }
+        s_cat[c][lane] = (double)acc;
+    }
+    __syncthreads();
+    double r0 = 0.0;
+    if (w == 0) {
+        double lh = 0.0;
+        for (int cc = 0; cc < ncat; cc++) lh += P.props[cc] * s_cat[cc][lane];
+        lh = fabs(lh) + P.invar[p];
+        const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
+        const double l = log(lh) + e * kLn2;
+        if (p < P.nptn) P.ptn_lh[p] = l;
+        r0 = wave_sum(p < P.nptn ? l * P.freq[p] : 0.0);
+    }
+    finish_tiles(P, r0, 0.0, tile, P.ntiles);
+}
+
+// ---------------------------------------------------------------------------------------
+// theta for the FP32 form, in FP64: theta_c[i] = (sum_x pi_x U[x][i] A_c[x]) *
+// (sum_y Uinv[i][y] B_c[y]), the eigen-space product the derivative kernels consume
+// (the reference's theta_all, tree/phylokernelnew.h:2187-2193), followed by the first
+// derivative evaluation.  buffer_scale_all = (e_A + e_B) ln 2.
+template <int N>
+__device__ inline void theta_f32_body(const KParams &P, const int tile) {
+    const int lane = threadIdx.x & 63;
+    const int w = wave_id();
+    const int ncw = nwaves(), ncat = P.ncat;
+    const int64_t p = (int64_t)tile * 64 + lane;
+    const bool atip = P.br_a < 0;
+    const int sa = atip ? P.states[(int64_t)(-P.br_a - 1) * P.nptn_pad + p] : 0;
+    const float *__restrict__ partf = (const float *)P.part;
+    double *__restrict__ theta = (double *)P.theta;
+    const cdptr piU = as_const(P.piU);
+    const cdptr uinv = as_const(P.inv_evec);
+    __shared__ double s_val[3][PLK_GPU_MAX_CATS * N];
+    __shared__ double s_cat[3][PLK_GPU_MAX_CATS][64];
+    for (int idx = threadIdx.x; idx < N * ncat; idx += blockDim.x) {
+        const int c = idx / N, i = idx % N;
+        const double cof = P.eval[i] * P.rates[c];
+        const double val = exp(cof * P.br_len) * P.props[c];
+        s_val[0][idx] = val;
+        s_val[1][idx] = cof * val;
+        s_val[2][idx] = cof * cof * val;
+    }
+    __syncthreads();
+    for (int c = w; c < ncat; c += ncw) {
+        const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+        double a[N], b[N];
+        const float *__restrict__ B = partf + (int64_t)P.br_b * P.slot_stride + off;
+        const float *__restrict__ A = partf + (int64_t)(atip ? 0 : P.br_a) * P.slot_stride + off;
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+            b[x] = (double)B[x * 64];
+            a[x] = atip ? P.tipvec[sa * N + x] : (double)A[x * 64];
+        }
+        double l0 = 0.0, l1 = 0.0, l2 = 0.0;
+        for_states<N>([&](const int i) {
+            double ai = 0.0, bi = 0.0;
+#pragma unroll
+            for (int x = 0; x < N; x++) {
+                ai = fma(piU[i * N + x], a[x], ai);
+                bi = fma(uinv[i * N + x], b[x], bi);
+            }
+            const double th = ai * bi;
+            theta[off + (int64_t)i * 64] = th;
+            l0 = fma(s_val[0][c * N + i], th, l0);
+            l1 = fma(s_val[1][c * N + i], th, l1);
+            l2 = fma(s_val[2][c * N + i], th, l2);
+        });
+        s_cat[0][c][lane] = l0;
+        s_cat[1][c][lane] = l1;
+        s_cat[2][c][lane] = l2;
+    }
+    __syncthreads();
+    double r0 = 0.0, r1 = 0.0;
+    if (w == 0) {
+        const int *expo = (const int *)P.scale;
+        const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
+        P.buf_scale[p] = e * kLn2;
+        double lh = 0.0, d1 = 0.0, d2 = 0.0;
+        for (int c = 0; c < ncat; c++) {
+            lh += s_cat[0][c][lane];
+            d1 += s_cat[1][c][lane];
+            d2 += s_cat[2][c][lane];
+        }
+        lh = fabs(lh) + P.invar[p];
+        const double f = (p < P.nptn) ? P.freq[p] : 0.0;
+        const double inv = 1.0 / lh;
+        const double dfr = d1 * inv, ddfr = d2 * inv;
+        const double t1 = dfr * f, t2 = ddfr * f;
+        r0 = wave_sum(p < P.nptn ? t1 : 0.0);
+        r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
+    }
+    finish_tiles(P, r0, r1, tile, P.ntiles);
+}
+
+}  // namespace dev
+}  // namespace plk

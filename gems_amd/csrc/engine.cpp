@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -42,7 +43,7 @@ static int64_t mat_stride_of(int n, int ncat, int su, int prec) {
     return round_up(2LL * ncat * n * n + 2LL * (su + 1) * n * ncat, 32);
 }
 
-static int alloc_stage(Ctx &c, size_t bytes) {
+int alloc_stage(Ctx &c, size_t bytes) {
     if (bytes <= c.h_stage_cap) return PLK_OK;
     if (c.h_stage) (void)hipHostFree(c.h_stage);
     c.h_stage = nullptr;
@@ -52,24 +53,27 @@ static int alloc_stage(Ctx &c, size_t bytes) {
     return PLK_OK;
 }
 
-static int ensure_op_capacity(Ctx &c, int nops) {
+// Stream-ordered allocations: hipFree would synchronise every stream of the device and
+// stall other contexts' work (the serving loop runs many contexts concurrently).
+int ensure_op_capacity(Ctx &c, int nops) {
     if (nops <= c.ops_cap && nops <= c.mats_cap) return PLK_OK;
     int cap = std::max(nops, std::max(16, c.ops_cap * 2));
-    if (c.d_ops) (void)hipFree(c.d_ops);
-    if (c.d_jobs) (void)hipFree(c.d_jobs);
-    if (c.d_mats) (void)hipFree(c.d_mats);
+    if (c.d_ops) (void)hipFreeAsync(c.d_ops, c.stream);
+    if (c.d_jobs) (void)hipFreeAsync(c.d_jobs, c.stream);
+    if (c.d_mats) (void)hipFreeAsync(c.d_mats, c.stream);
     c.d_ops = nullptr;
     c.d_jobs = nullptr;
     c.d_mats = nullptr;
     c.ops_cap = c.mats_cap = 0;
-    HIP_TRY(hipMalloc(&c.d_ops, sizeof(DevOp) * cap));
-    HIP_TRY(hipMalloc(&c.d_jobs, sizeof(PJob) * 2 * cap));
-    HIP_TRY(hipMalloc(&c.d_mats, c.real_size() * c.mat_stride * cap));
+    HIP_TRY(hipMallocAsync((void **)&c.d_ops, sizeof(DevOp) * cap, c.stream));
+    HIP_TRY(hipMallocAsync((void **)&c.d_jobs, sizeof(PJob) * 2 * cap, c.stream));
+    HIP_TRY(hipMallocAsync(&c.d_mats, c.real_size() * c.mat_stride * cap, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));  // usable from any stream (batched launches)
     c.ops_cap = c.mats_cap = cap;
     return PLK_OK;
 }
 
-static KParams make_params(Ctx &c) {
+KParams make_params(Ctx &c) {
     KParams p{};
     p.n = c.n;
     p.ncat = c.ncat;
@@ -186,7 +190,7 @@ static int check_ctx(plk_ctx *ctx) {
     return PLK_OK;
 }
 
-static bool valid_ref(const Ctx &c, int r) { return r >= 0 ? r < c.nslots : (-r - 1) < c.ntaxa; }
+bool valid_ref(const Ctx &c, int r) { return r >= 0 ? r < c.nslots : (-r - 1) < c.ntaxa; }
 
 }  // namespace plk
 
@@ -230,6 +234,17 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     auto *c = new plk_ctx();
     int dev = 0;
     (void)hipGetDevice(&dev);
+    {
+        // keep freed stream-ordered memory in the device pool for the next context
+        static std::once_flag pool_once;
+        std::call_once(pool_once, [dev] {
+            hipMemPool_t pool = nullptr;
+            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+                uint64_t thr = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+            }
+        });
+    }
     c->dev = dev;
     c->n = nstates;
     c->ncat = ncat;
@@ -260,20 +275,20 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
                                 std::string("plk_create: ") + #expr + ": " + hipGetErrorString(e_))); \
     } while (0)
     CREATE_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    CREATE_TRY(hipMalloc(&c->d_part, rs * slot * nslots));
-    CREATE_TRY(hipMalloc(&c->d_scale, 2 * scale_slot * nslots));
-    CREATE_TRY(hipMalloc(&c->d_states, (size_t)ntaxa * c->nptn_pad));
-    CREATE_TRY(hipMalloc(&c->d_freq, sizeof(double) * c->nptn_pad));
-    CREATE_TRY(hipMalloc(&c->d_invar, sizeof(double) * c->nptn_pad));
-    CREATE_TRY(hipMalloc(&c->d_theta, sizeof(double) * slot));
-    CREATE_TRY(hipMalloc(&c->d_buf_scale, sizeof(double) * c->nptn_pad));
-    CREATE_TRY(hipMalloc(&c->d_ptn_lh, sizeof(double) * c->nptn_pad));
-    CREATE_TRY(hipMalloc(&c->d_tile_sums, sizeof(double) * 2 * c->ntiles));
-    CREATE_TRY(hipMalloc(&c->d_result, sizeof(double) * 2));
-    CREATE_TRY(hipMalloc(&c->d_counter, sizeof(unsigned)));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_part, rs * slot * nslots, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_scale, 2 * scale_slot * nslots, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_states, (size_t)ntaxa * c->nptn_pad, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_freq, sizeof(double) * c->nptn_pad, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_invar, sizeof(double) * c->nptn_pad, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_theta, sizeof(double) * slot, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_buf_scale, sizeof(double) * c->nptn_pad, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_ptn_lh, sizeof(double) * c->nptn_pad, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_tile_sums, sizeof(double) * 2 * c->ntiles, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_result, sizeof(double) * 2, c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_counter, sizeof(unsigned), c->stream));
     const int n = nstates;
     const size_t model_doubles = n + 2 * n * n + 2 * ncat + 2 * (size_t)(c->su + 1) * n + n + n * n;
-    CREATE_TRY(hipMalloc(&c->d_model, sizeof(double) * model_doubles));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_model, sizeof(double) * model_doubles, c->stream));
     c->d_eval = c->d_model;
     c->d_evec = c->d_eval + n;
     c->d_inv_evec = c->d_evec + n * n;
@@ -305,7 +320,8 @@ extern "C" int plk_destroy(plk_ctx *c) {
                     c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
                     c->d_result, c->d_counter};
     for (void *b : bufs)
-        if (b) (void)hipFree(b);
+        if (b) (void)hipFreeAsync(b, c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_result) (void)hipHostFree(c->h_result);
     for (int i = 0; i < 6; i++)

@@ -116,4 +116,18 @@ hipError_t f64_theta_eval(const KParams &p, int deriv, hipStream_t s);
 hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
 hipError_t f32_traverse(const KParams &p, int mode, hipStream_t s);
 hipError_t f32_theta(const KParams &p, hipStream_t s);
+
+// Shared host helpers (engine.cpp).
+KParams make_params(Ctx &c);
+bool valid_ref(const Ctx &c, int r);
+int ensure_op_capacity(Ctx &c, int nops);
+
+// Batched partitions: one work item = one 64-pattern tile of one partition.
+struct BatchItem {
+    int32_t part;
+    int32_t tile;
+};
+hipError_t batch_pmatrix(int prec, const KParams *params, const PJob *jobs, int njobs, hipStream_t s);
+hipError_t batch_traverse(int prec, const KParams *params, const BatchItem *items, int nitems, int grid,
+                          hipStream_t s);
 }  // namespace plk

@@ -1,0 +1,84 @@
+// Batched unlinked partitions (IQ-TREE's PhyloSuperTree loop over partitions,
+// tree/phylosupertree.cpp:662-734) in one launch per traversal: every work item is one
+// 64-pattern tile of one partition; each partition keeps its own model, P(t), slots,
+// scale counters and fixed-order reduction, so its result is bit-identical to running
+// it alone.  DNA and protein partitions share the launch (the state count is a
+// per-item branch, uniform within a workgroup).
+#include "dev_f32.h"
+#include "dev_f64.h"
+
+namespace plk {
+namespace {
+using namespace dev;
+
+// Copy a wave-uniform struct out of device memory through the constant address space
+// (scalar loads).
+template <typename T>
+__device__ __forceinline__ T load_uniform(const T *p) {
+    static_assert(sizeof(T) % 8 == 0, "8-byte granular structs only");
+    typedef const __attribute__((address_space(4))) uint64_t *cu64;
+    T v;
+    uint64_t *dst = reinterpret_cast<uint64_t *>(&v);
+    const cu64 src = (cu64)p;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 8); i++) dst[i] = src[i];
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_batch_pmatrix_f64(const KParams *params, const PJob *jobs) {
+    const PJob j = load_uniform(jobs + blockIdx.x);
+    const KParams P = load_uniform(params + j.pad);
+    pmatrix_f64_body(P, j);
+}
+
+__global__ void __launch_bounds__(256) k_batch_pmatrix_f32(const KParams *params, const PJob *jobs) {
+    const PJob j = load_uniform(jobs + blockIdx.x);
+    const KParams P = load_uniform(params + j.pad);
+    switch (P.n) {
+        case 4: pmatrix_f32_body<4>(P, j); break;
+        case 20: pmatrix_f32_body<20>(P, j); break;
+        default: break;
+    }
+}
+
+// Persistent when gridDim < nitems: workgroup g takes items g, g + grid, ... (the host
+// orders items by decreasing cost, so the static stride is a longest-first schedule).
+__global__ void __launch_bounds__(256) k_batch_traverse_f64(const KParams *params, const BatchItem *items, int nitems) {
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        const BatchItem it = load_uniform(items + i);
+        const KParams P = load_uniform(params + it.part);
+        if (P.n == 4) traverse_f64_body<4, MODE_LOGL>(P, it.tile);
+        else traverse_f64_body<20, MODE_LOGL>(P, it.tile);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256) k_batch_traverse_f32(const KParams *params, const BatchItem *items, int nitems) {
+    for (int i = blockIdx.x; i < nitems; i += gridDim.x) {
+        const BatchItem it = load_uniform(items + i);
+        const KParams P = load_uniform(params + it.part);
+        if (P.n == 4) traverse_f32_valu_body<4, MODE_LOGL>(P, it.tile);
+        else traverse_f32_mfma_body<20, MODE_LOGL>(P, it.tile);
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t batch_pmatrix(int prec, const KParams *params, const PJob *jobs, int njobs, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    if (prec == PLK_FP64) hipLaunchKernelGGL(k_batch_pmatrix_f64, dim3(njobs), dim3(256), 0, s, params, jobs);
+    else hipLaunchKernelGGL(k_batch_pmatrix_f32, dim3(njobs), dim3(256), 0, s, params, jobs);
+    return hipGetLastError();
+}
+
+hipError_t batch_traverse(int prec, const KParams *params, const BatchItem *items, int nitems, int grid,
+                          hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    const int g = (grid <= 0 || grid > nitems) ? nitems : grid;
+    if (prec == PLK_FP64) hipLaunchKernelGGL(k_batch_traverse_f64, dim3(g), dim3(256), 0, s, params, items, nitems);
+    else hipLaunchKernelGGL(k_batch_traverse_f32, dim3(g), dim3(256), 0, s, params, items, nitems);
+    return hipGetLastError();
+}
+
+}  // namespace plk

@@ -1,449 +1,22 @@
-// FP64 likelihood kernels for gfx950 (reference semantics, eigen-space partials).
-//
-// Layout: a tile is 64 consecutive patterns, one wavefront lane per pattern; a
-// workgroup owns one tile and has ncw = min(ncat, 4) wavefronts, wave w handling the
-// categories c = w, w + ncw, ...  Element (ptn, c, x) of a partial is at
-// ((tile*ncat + c)*n + x)*64 + lane, so every load/store of one (c, x) by a wavefront
-// is one contiguous 512-byte segment.  All tiles are independent across the whole
-// traversal (the reference's packets walk every node for their pattern slice,
-// tree/phylokernelnew.h:2851-2853), so one launch runs the full post-order op list with
-// no grid-wide barrier; each thread only re-reads values it wrote itself.
-#include <hip/hip_runtime.h>
-
-#include <type_traits>
-
-#include "engine_common.h"
+// FP64 kernels for gfx950: thin __global__ wrappers over the bodies in dev_f64.h.
+#include "dev_f64.h"
 
 namespace plk {
 namespace {
+using namespace dev;
 
-constexpr double kST = 8.636168555094444625386e-78;  // 2^-256
-constexpr double kTwo256 = 0x1p256;
-constexpr double kLogST = -177.4456782233459932741;
-
-__device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
-
-// Read-only view in the constant address space: wave-uniform reads through it compile to
-// scalar loads whose results feed the FMAs as SGPR operands (the matrices are written
-// by an earlier launch only).
-typedef const __attribute__((address_space(4))) double *cdptr;
-__device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
-
-// Compile-time loop: f(I) for I = B..E-1, fully unrolled.
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for(F &&f) {
-    if constexpr (B < E) {
-        f(std::integral_constant<int, B>{});
-        static_for<B + 1, E>(f);
-    }
-}
-
-// x-loop over output states: unrolled for small state counts, a plain loop for the
-// 61-state codon model (whose fast path is the FP32 matrix-core form).
-template <int N, typename F>
-__device__ __forceinline__ void for_states(F &&f) {
-    if constexpr (N <= 20) {
-        static_for<0, N>([&](auto X) { f(decltype(X)::value); });
-    } else {
-#pragma unroll 1
-        for (int x = 0; x < N; x++) f(x);
-    }
-}
-
-// Deterministic wavefront sum (fixed shuffle tree); result valid in lane 0.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
-}
-
-// Per-tile values -> fixed-order grid total.  The last workgroup to finish sums the
-// tile values in index order, so the result does not depend on scheduling; it then
-// re-arms the counter for the next launch (no memset needed, graph-safe).
-__device__ void finish_tiles(const KParams &P, double v0, double v1) {
-    __shared__ int s_last;
-    if (threadIdx.x == 0) {
-        P.tile_sums[2 * blockIdx.x] = v0;
-        P.tile_sums[2 * blockIdx.x + 1] = v1;
-        __threadfence();
-        unsigned prev = atomicAdd(P.counter, 1u);
-        s_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    if (threadIdx.x < 64) {
-        double a0 = 0.0, a1 = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += 64) {
-            a0 += __hip_atomic_load(&P.tile_sums[2 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a1 += __hip_atomic_load(&P.tile_sums[2 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        a0 = wave_sum(a0);
-        a1 = wave_sum(a1);
-        if (threadIdx.x == 0) {
-            P.result[0] = a0;
-            P.result[1] = a1;
-            __hip_atomic_store(P.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------------------
-// P(t) kernel (computePartialInfo, tree/phylokernelnew.h:1013-1063): for one child of one
-// op, echild[c][x][i] = U[x][i] exp(lambda_i r_c t), and for a leaf child the table
-// leaf[s][c][x] = sum_i echild[c][x][i] tip_lh[s][i] with the unknown-state row = 1.
 __global__ void __launch_bounds__(256) k_pmatrix_f64(KParams P, const PJob *__restrict__ jobs) {
-    const PJob j = jobs[blockIdx.x];
-    const int n = P.n, ncat = P.ncat, block = n * ncat, su = P.su;
-    double *M = (double *)P.mats + (int64_t)j.op * P.mat_stride;
-    double *E = M + (int64_t)j.side * ncat * n * n;
-    __shared__ double ex[PLK_GPU_MAX_CATS * PLK_MAX_STATES];
-    for (int idx = threadIdx.x; idx < ncat * n; idx += blockDim.x) {
-        int c = idx / n, i = idx % n;
-        ex[idx] = exp(P.eval[i] * (P.rates[c] * j.len));
-    }
-    __syncthreads();
-    for (int idx = threadIdx.x; idx < ncat * n * n; idx += blockDim.x) {
-        int c = idx / (n * n), rem = idx % (n * n), x = rem / n, i = rem % n;
-        E[idx] = P.evec[x * n + i] * ex[c * n + i];
-    }
-    if (j.leaf) {
-        double *T = M + (int64_t)2 * ncat * n * n + (int64_t)j.side * (su + 1) * block;
-        for (int idx = threadIdx.x; idx < su * block; idx += blockDim.x) {
-            int s = idx / block, rem = idx % block, c = rem / n, x = rem % n;
-            double v = 0.0;
-            for (int i = 0; i < n; i++) v = fma(P.evec[x * n + i] * ex[c * n + i], P.tip_lh[s * n + i], v);
-            T[idx] = v;
-        }
-        for (int idx = threadIdx.x; idx < block; idx += blockDim.x) T[(int64_t)su * block + idx] = 1.0;
-    }
+    pmatrix_f64_body(P, jobs[blockIdx.x]);
 }
-
-// ---------------------------------------------------------------------------------------
-// One (pattern, category) of one op: tmp[x] = (eL[x].L) * (eR[x].R) with tip children
-// replaced by their leaf-table rows, then out = Uinv tmp
-// (tree/phylokernelnew.h:1729-1747, 1848-1890, 1979-1994).  Matrices are wave-uniform
-// (scalar loads); partials are coalesced 64-lane loads.
-template <int N>
-__device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, const double *__restrict__ M, int c,
-                                           int64_t off, int sl, int sr, double (&out)[N]) {
-    const int ncat = P.ncat, block = N * ncat;
-    const double *__restrict__ part = (const double *)P.part;
-    const cdptr uinv = as_const(P.inv_evec);
-    const cdptr eL = as_const(M + (int64_t)c * N * N);
-    const cdptr eR = as_const(M + (int64_t)(ncat + c) * N * N);
-    const double *__restrict__ tabs = M + (int64_t)2 * ncat * N * N;
-#pragma unroll
-    for (int j = 0; j < N; j++) out[j] = 0.0;
-    if (op.kind == KIND_II) {
-        const double *__restrict__ L = part + (int64_t)op.left * P.slot_stride + off;
-        const double *__restrict__ R = part + (int64_t)op.right * P.slot_stride + off;
-        double l[N], r[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            l[i] = L[i * 64];
-            r[i] = R[i * 64];
-        }
-        for_states<N>([&](const int x) {
-            double a = 0.0, b = 0.0;
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                a = fma(eL[x * N + i], l[i], a);
-                b = fma(eR[x * N + i], r[i], b);
-            }
-            const double t = a * b;
-#pragma unroll
-            for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);
-        });
-    } else if (op.kind == KIND_TI) {
-        const double *__restrict__ R = part + (int64_t)op.right * P.slot_stride + off;
-        const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
-        double r[N];
-#pragma unroll
-        for (int i = 0; i < N; i++) r[i] = R[i * 64];
-        for_states<N>([&](const int x) {
-            double b = 0.0;
-#pragma unroll
-            for (int i = 0; i < N; i++) b = fma(eR[x * N + i], r[i], b);
-            const double t = tl[x] * b;
-#pragma unroll
-            for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);
-        });
-    } else {
-        const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
-        const double *__restrict__ tr = tabs + (int64_t)(P.su + 1) * block + (int64_t)sr * block + c * N;
-        for_states<N>([&](const int x) {
-            const double t = tl[x] * tr[x];
-#pragma unroll
-            for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);
-        });
-    }
-}
-
-// Fill the branch vectors val0/1/2[c][i] = (1, cof, cof^2) * exp(cof t) * prop_c with
-// cof = lambda_i r_c (tree/phylokernelnew.h:2321-2367, 2731-2766).
-__device__ __forceinline__ void fill_val(const KParams &P, double *v0, double *v1, double *v2) {
-    const int n = P.n, block = n * P.ncat;
-    for (int idx = threadIdx.x; idx < block; idx += blockDim.x) {
-        int c = idx / n, i = idx % n;
-        double cof = P.eval[i] * P.rates[c];
-        double val = exp(cof * P.br_len) * P.props[c];
-        v0[idx] = val;
-        if (v1) {
-            double val1 = cof * val;
-            v1[idx] = val1;
-            v2[idx] = cof * val1;
-        }
-    }
-}
-
-// SAFE per-category factor relative to the minimum scale (2918-2944): 1, 2^-256 or 0.
-__device__ __forceinline__ double safe_factor(int d) { return d == 0 ? 1.0 : (d == 1 ? kST : 0.0); }
 
 template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_traverse_f64(KParams P) {
-    const int lane = threadIdx.x & 63;
-    const int w = wave_id();
-    const int ncw = P.ncw, ncat = P.ncat;
-    const int tile = blockIdx.x;
-    const int64_t p = (int64_t)tile * 64 + lane;
-    double *__restrict__ part = (double *)P.part;
-    uint16_t *__restrict__ scale = (uint16_t *)P.scale;
-    const double invar = P.invar[p];
-    __shared__ double s_max[2][4][64];
-    __shared__ double s_val[3][PLK_GPU_MAX_CATS * N];
-    __shared__ double s_cat[3][PLK_GPU_MAX_CATS][64];
-    __shared__ int s_sc[PLK_GPU_MAX_CATS][64];
-
-    if (MODE != MODE_PARTIALS) fill_val(P, s_val[0], MODE == MODE_THETA ? s_val[1] : nullptr, s_val[2]);
-
-    // -------- post-order traversal (computePartialLikelihood for every op)
-    for (int k = 0; k < P.nops; k++) {
-        const DevOp op = P.ops[k];
-        const double *__restrict__ M = (const double *)P.mats + (int64_t)k * P.mat_stride;
-        double *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride;
-        int sl = 0, sr = 0;
-        if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
-        if (op.kind == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
-        double wmax = 0.0;
-        for (int c = w; c < ncat; c += ncw) {
-            const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
-            double out[N];
-            op_partial<N>(P, op, M, c, off, sl, sr, out);
-            double m = 0.0;
-#pragma unroll
-            for (int j = 0; j < N; j++) m = fmax(m, fabs(out[j]));
-            if (P.safe) {
-                // per-category counts and check (1950-2013, 1756-1760, 1871-1886)
-                const int64_t a = p * ncat + c;
-                unsigned sc = 0;
-                if (op.kind == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + a];
-                else if (op.kind == KIND_II)
-                    sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + a] + scale[(int64_t)op.right * P.scale_stride + a];
-                if (op.kind != KIND_TT && m < kST && invar == 0.0) {
-#pragma unroll
-                    for (int j = 0; j < N; j++) out[j] *= kTwo256;
-                    sc += 1;
-                }
-                scale[(int64_t)op.out_slot * P.scale_stride + a] = (uint16_t)sc;
-            }
-            wmax = fmax(wmax, m);
-#pragma unroll
-            for (int j = 0; j < N; j++) outp[off + j * 64] = out[j];
-        }
-        if (!P.safe) {
-            // one check over all categories of the pattern (2019-2034, 1893-1907)
-            s_max[k & 1][w][lane] = wmax;
-            __syncthreads();
-            double pm = 0.0;
-            for (int ww = 0; ww < ncw; ww++) pm = fmax(pm, s_max[k & 1][ww][lane]);
-            const bool do_scale = (op.kind != KIND_TT) && pm < kST && invar == 0.0;
-            if (do_scale) {
-                for (int c = w; c < ncat; c += ncw) {
-                    const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
-#pragma unroll
-                    for (int j = 0; j < N; j++) outp[off + j * 64] *= kTwo256;
-                }
-            }
-            if (w == 0) {
-                unsigned sc = 0;
-                if (op.kind == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + p];
-                else if (op.kind == KIND_II)
-                    sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + p] + scale[(int64_t)op.right * P.scale_stride + p];
-                scale[(int64_t)op.out_slot * P.scale_stride + p] = (uint16_t)(sc + (do_scale ? 1u : 0u));
-            }
-        }
-    }
-    if (MODE == MODE_PARTIALS) return;
-
-    // -------- branch stage: node refs a (slot or tip) and b (slot)
-    __syncthreads();  // s_val complete
-    const bool atip = P.br_a < 0;
-    const int sa = atip ? P.states[(int64_t)(-P.br_a - 1) * P.nptn_pad + p] : 0;
-    const double *__restrict__ A = atip ? nullptr : part + (int64_t)P.br_a * P.slot_stride;
-    const double *__restrict__ B = part + (int64_t)P.br_b * P.slot_stride;
-    const uint16_t *__restrict__ scA = atip ? nullptr : scale + (int64_t)P.br_a * P.scale_stride;
-    const uint16_t *__restrict__ scB = scale + (int64_t)P.br_b * P.scale_stride;
-    const double *__restrict__ tip = P.tip_lh + sa * N;
-    double *__restrict__ theta = (double *)P.theta;
-
-    for (int c = w; c < ncat; c += ncw) {
-        const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
-        const double *__restrict__ v0 = s_val[0] + c * N;
-        if (MODE == MODE_LOGL) {
-            // lh_cat = sum_i val * node * dad (dotProduct3Vec 3044-3057; tip 2903-2914)
-            double acc = 0.0;
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const double a = atip ? tip[i] : A[off + i * 64];
-                acc = fma(v0[i] * a, B[off + i * 64], acc);
-            }
-            s_cat[0][c][lane] = acc;
-        } else {
-            // theta = node * dad (2187-2193; tip 2104-2145), then the triple dot (2499-2501)
-            const double *__restrict__ v1 = s_val[1] + c * N;
-            const double *__restrict__ v2 = s_val[2] + c * N;
-            double l0 = 0.0, l1 = 0.0, l2 = 0.0;
-#pragma unroll
-            for (int i = 0; i < N; i++) {
-                const double a = atip ? tip[i] : A[off + i * 64];
-                const double th = a * B[off + i * 64];
-                theta[off + i * 64] = th;
-                l0 = fma(v0[i], th, l0);
-                l1 = fma(v1[i], th, l1);
-                l2 = fma(v2[i], th, l2);
-            }
-            s_cat[0][c][lane] = l0;
-            s_cat[1][c][lane] = l1;
-            s_cat[2][c][lane] = l2;
-        }
-        if (P.safe) s_sc[c][lane] = (int)scB[p * ncat + c] + (atip ? 0 : (int)scA[p * ncat + c]);
-    }
-    __syncthreads();
-
-    int sc_min = 0;
-    if (P.safe) {
-        sc_min = s_sc[0][lane];
-        for (int c = 1; c < ncat; c++) sc_min = min(sc_min, s_sc[c][lane]);
-        if (MODE == MODE_THETA) {
-            // rescale the stored theta of categories above the minimum (2146-2171, 2194-2225)
-            for (int c = w; c < ncat; c += ncw) {
-                const int d = s_sc[c][lane] - sc_min;
-                if (d != 0) {
-                    const double f = safe_factor(d);
-                    const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
-#pragma unroll
-                    for (int i = 0; i < N; i++) theta[off + i * 64] *= f;
-                }
-            }
-        }
-    }
-    double r0 = 0.0, r1 = 0.0;
-    if (w == 0) {
-        double sc;
-        double lh = 0.0, d1 = 0.0, d2 = 0.0;
-        if (P.safe) {
-            sc = sc_min;
-            for (int c = 0; c < ncat; c++) {
-                const double f = safe_factor(s_sc[c][lane] - sc_min);
-                lh += s_cat[0][c][lane] * f;
-                if (MODE == MODE_THETA) {
-                    d1 += s_cat[1][c][lane] * f;
-                    d2 += s_cat[2][c][lane] * f;
-                }
-            }
-        } else {
-            sc = (double)((unsigned)scB[p] + (atip ? 0u : (unsigned)scA[p]));
-            for (int c = 0; c < ncat; c++) {
-                lh += s_cat[0][c][lane];
-                if (MODE == MODE_THETA) {
-                    d1 += s_cat[1][c][lane];
-                    d2 += s_cat[2][c][lane];
-                }
-            }
-        }
-        lh = fabs(lh) + invar;
-        const double f = (p < P.nptn) ? P.freq[p] : 0.0;
-        if (MODE == MODE_LOGL) {
-            // 3096-3108
-            const double l = log(lh) + sc * kLogST;
-            if (p < P.nptn) P.ptn_lh[p] = l;
-            r0 = wave_sum(p < P.nptn ? l * f : 0.0);
-        } else {
-            // buffer_scale_all (2178-2179, 2231-2232) and df/ddf (2504-2515)
-            P.buf_scale[p] = sc * kLogST;
-            const double inv = 1.0 / lh;
-            const double dfr = d1 * inv, ddfr = d2 * inv;
-            const double t1 = dfr * f, t2 = ddfr * f;
-            r0 = wave_sum(p < P.nptn ? t1 : 0.0);
-            r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
-        }
-    }
-    finish_tiles(P, r0, r1);
+    traverse_f64_body<N, MODE>(P, blockIdx.x);
 }
 
-// Newton steps and logL from the resident theta buffer: derivatives
-// (dotProductTriple, tree/phylokernelnew.h:2474-2550) or logL
-// (computeLikelihoodFromBufferSIMD, 3373-3400).
 template <int N, int DERIV>
 __global__ void __launch_bounds__(256) k_theta_eval_f64(KParams P) {
-    const int lane = threadIdx.x & 63;
-    const int w = wave_id();
-    const int ncw = P.ncw, ncat = P.ncat;
-    const int tile = blockIdx.x;
-    const int64_t p = (int64_t)tile * 64 + lane;
-    const double *__restrict__ theta = (const double *)P.theta;
-    __shared__ double s_val[3][PLK_GPU_MAX_CATS * N];
-    __shared__ double s_cat[3][PLK_GPU_MAX_CATS][64];
-    fill_val(P, s_val[0], DERIV ? s_val[1] : nullptr, s_val[2]);
-    __syncthreads();
-    for (int c = w; c < ncat; c += ncw) {
-        const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
-        const double *__restrict__ v0 = s_val[0] + c * N;
-        const double *__restrict__ v1 = s_val[1] + c * N;
-        const double *__restrict__ v2 = s_val[2] + c * N;
-        double l0 = 0.0, l1 = 0.0, l2 = 0.0;
-#pragma unroll
-        for (int i = 0; i < N; i++) {
-            const double th = theta[off + i * 64];
-            l0 = fma(v0[i], th, l0);
-            if (DERIV) {
-                l1 = fma(v1[i], th, l1);
-                l2 = fma(v2[i], th, l2);
-            }
-        }
-        s_cat[0][c][lane] = l0;
-        if (DERIV) {
-            s_cat[1][c][lane] = l1;
-            s_cat[2][c][lane] = l2;
-        }
-    }
-    __syncthreads();
-    double r0 = 0.0, r1 = 0.0;
-    if (w == 0) {
-        double lh = 0.0, d1 = 0.0, d2 = 0.0;
-        for (int c = 0; c < ncat; c++) {
-            lh += s_cat[0][c][lane];
-            if (DERIV) {
-                d1 += s_cat[1][c][lane];
-                d2 += s_cat[2][c][lane];
-            }
-        }
-        lh = fabs(lh) + P.invar[p];
-        const double f = (p < P.nptn) ? P.freq[p] : 0.0;
-        if (DERIV) {
-            const double inv = 1.0 / lh;
-            const double dfr = d1 * inv, ddfr = d2 * inv;
-            const double t1 = dfr * f, t2 = ddfr * f;
-            r0 = wave_sum(p < P.nptn ? t1 : 0.0);
-            r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
-        } else {
-            const double l = log(fabs(lh)) + P.buf_scale[p];
-            if (p < P.nptn) P.ptn_lh[p] = l;
-            r0 = wave_sum(p < P.nptn ? l * f : 0.0);
-        }
-    }
-    finish_tiles(P, r0, r1);
+    theta_eval_f64_body<N, DERIV>(P, blockIdx.x);
 }
 
 template <int N>

@@ -438,3 +438,14 @@ extern "C" int plk_tree_counters(plk_tree *t, int64_t *out3) {
     out3[2] = t->n_ops;
     return PLK_OK;
 }
+
+extern "C" int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, int *ref_a, int *ref_b, double *len) {
+    if (!node_ok(t, dad, node) || !ref_a || !ref_b || !len) return fail(PLK_ERR_ARG, "plk_tree_queue_branch: bad arguments");
+    if (clear) t->clear_all();
+    int rc = t->plan(dad, node);
+    if (rc) return rc;
+    *ref_a = t->ref(dad);
+    *ref_b = t->ref(node);
+    *len = t->length(dad, node);
+    return PLK_OK;
+}

@@ -68,8 +68,8 @@ struct plk_problem {
     std::vector<uint8_t> states;
 };
 
-extern "C" int plk_sim_create(plk_problem **out, int seq_type, int ntaxa, int nsites, double mean_len,
-                              double alpha, int ncat, uint64_t seed) {
+static int sim_create(plk_problem **out, int seq_type, int ntaxa, const int32_t *given_nei, int nsites,
+                      double mean_len, double alpha, int ncat, uint64_t seed) {
     if (!out || ntaxa < 3 || ntaxa > 100000 || nsites < 1 || !(mean_len > 0) || ncat < 1 || ncat > PLK_MAX_CATS)
         return plk::fail(PLK_ERR_ARG, "plk_sim_create: bad arguments");
     if (seq_type < PLK_SEQ_DNA || seq_type > PLK_SEQ_CODON) return plk::fail(PLK_ERR_ARG, "plk_sim_create: bad seq_type");
@@ -154,6 +154,32 @@ extern "C" int plk_sim_create(plk_problem **out, int seq_type, int ntaxa, int ns
         p->blen[a * 3 + ka] = p->blen[b * 3 + kb] = len;
     };
     for (auto &e : edges) link(id[e.a], id[e.b], std::max(e.len, 1e-6));
+    if (given_nei) {
+        // caller's topology, fresh lengths (the random Yule draw above keeps the
+        // generator stream identical between the two entry points)
+        p->nei.assign(given_nei, given_nei + (size_t)nn * 3);
+        p->blen.assign((size_t)nn * 3, 0.0);
+        for (int u = 0; u < nn; u++)
+            for (int k = 0; k < 3; k++) {
+                int v = p->nei[u * 3 + k];
+                if (v < 0) continue;
+                if (v < 0 || v >= nn) {
+                    delete p;
+                    return plk::fail(PLK_ERR_ARG, "plk_sim_create_on_tree: bad topology");
+                }
+                if (v > u) {
+                    double len = std::max(rng.exponential(mean_len), 1e-6);
+                    p->blen[u * 3 + k] = len;
+                    int kb = 0;
+                    while (kb < 3 && p->nei[v * 3 + kb] != u) kb++;
+                    if (kb == 3) {
+                        delete p;
+                        return plk::fail(PLK_ERR_ARG, "plk_sim_create_on_tree: topology not symmetric");
+                    }
+                    p->blen[v * 3 + kb] = len;
+                }
+            }
+    }
 
     // --- simulate sites down the tree from the first internal node
     std::vector<std::vector<uint8_t>> seq(nn, std::vector<uint8_t>(nsites));
@@ -220,6 +246,17 @@ extern "C" int plk_sim_create(plk_problem **out, int seq_type, int ntaxa, int ns
         for (int k = 0; k < p->nptn; k++) p->states[(size_t)t * p->nptn + k] = seq[t][col_of[k]];
     *out = p;
     return PLK_OK;
+}
+
+extern "C" int plk_sim_create(plk_problem **out, int seq_type, int ntaxa, int nsites, double mean_len,
+                              double alpha, int ncat, uint64_t seed) {
+    return sim_create(out, seq_type, ntaxa, nullptr, nsites, mean_len, alpha, ncat, seed);
+}
+
+extern "C" int plk_sim_create_on_tree(plk_problem **out, int seq_type, int ntaxa, const int32_t *nei, int nsites,
+                                      double mean_len, double alpha, int ncat, uint64_t seed) {
+    if (!nei) return plk::fail(PLK_ERR_ARG, "plk_sim_create_on_tree: null topology");
+    return sim_create(out, seq_type, ntaxa, nei, nsites, mean_len, alpha, ncat, seed);
 }
 
 extern "C" int plk_sim_destroy(plk_problem *p) {

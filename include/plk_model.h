@@ -56,6 +56,10 @@ int plk_codon_exch(double kappa, double omega, double *exch);
 typedef struct plk_problem plk_problem;
 int plk_sim_create(plk_problem **out, int seq_type, int ntaxa, int nsites, double mean_len,
                    double alpha, int ncat, uint64_t seed);
+/* Same, on a given unrooted topology nei ((2*ntaxa-2) x 3, leaves 0..ntaxa-1 with their
+ * neighbour in slot 0) with fresh Exp(mean_len) branch lengths (unlinked partitions). */
+int plk_sim_create_on_tree(plk_problem **out, int seq_type, int ntaxa, const int32_t *nei, int nsites,
+                           double mean_len, double alpha, int ncat, uint64_t seed);
 int plk_sim_destroy(plk_problem *p);
 /* dims[0..6] = nstates, ntaxa, nsites, nptn, ncat, state_unknown, seq_type */
 int plk_sim_dims(const plk_problem *p, int *dims);

@@ -52,6 +52,10 @@ int plk_tree_optimize_all_branches(plk_tree *t, int iterations, double tol, doub
                                    int max_nr_step, double *logl);
 /* Invalidate every partial and run one full traversal + branch logL (the benchmark step). */
 int plk_tree_full_traversal(plk_tree *t, double *logl);
+/* Plan the traversal of branch (dad, node) and queue its ops on the tree's context without
+ * evaluating it; returns the node references and length for plk_batch_loglik.  With
+ * clear != 0 every partial is invalidated first (a full traversal). */
+int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, int *ref_a, int *ref_b, double *len);
 /* Counters: [0] partial updates (site x node), [1] derivative evaluations, [2] traversal ops */
 int plk_tree_counters(plk_tree *t, int64_t *out3);
 
