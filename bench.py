@@ -84,6 +84,9 @@ def main():
 
     def run_form(precision, label):
         g = G.Tree(p, precision=precision, scaling=G.NORM)
+        tile = 1
+        if precision == G.FP32:
+            tile, _ = g.autotune(reps=2)  # matrix-core tile shape (waves per category)
         for _ in range(args.warmup):
             g.full_traversal()
         g.set_timing(True)
@@ -104,7 +107,10 @@ def main():
         log(f"{label}: {el / args.steps * 1e3:.3f} ms/step, kernel {k_avg_s * 1e3:.3f} ms, {ups / 1e9:.3f} G upd/s, "
             f"{achieved:.0f} GB/s algorithmic, logL {logl:.6f}")
         return {"ups": ups, "ms_per_step": el / args.steps * 1e3, "kernel_avg_ms": k_avg_s * 1e3,
-                "achieved": achieved, "bytes_per_update": bpu, "logl": logl}
+                "achieved": achieved, "bytes_per_update": bpu, "logl": logl, "tile_waves_per_category": tile}
+
+    copy_gbps = G.probe_copy(1 << 30, 5)
+    log(f"measured device copy bandwidth: {copy_gbps:.0f} GB/s")
 
     f64 = run_form(G.FP64, "fp64")
     f32 = run_form(G.FP32, "fp32")
@@ -149,6 +155,8 @@ def main():
             "frac": f32["achieved"] / HBM_PEAK_GBS, "traffic": None,
             "kernel": "k_traverse_f32_mfma<20,MODE_LOGL>", "kernel_avg_ms": f32["kernel_avg_ms"],
             "alg_bytes_per_update": f32["bytes_per_update"],
+            "measured_copy_GBps": copy_gbps, "frac_of_measured_copy": f32["achieved"] / copy_gbps,
+            "tile_waves_per_category": f32["tile_waves_per_category"],
         },
         "cpu_baseline": cpu,
     }

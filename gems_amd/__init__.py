@@ -80,6 +80,8 @@ _SIGS = {
     "plk_updates": (ctypes.c_int64, [_P]),
     "plk_synchronize": (_int, [_P]),
     "plk_get_exponents": (_int, [_P, _int, _ip]),
+    "plk_set_tile_shape": (_int, [_P, _int]),
+    "plk_probe_copy": (_int, [ctypes.c_int64, _int, _dp]),
     # model / synthetic data
     "plk_state_unknown": (_int, [_int, _int]),
     "plk_rate_matrix": (_int, [_int, _dp, _dp, _dp]),
@@ -110,6 +112,7 @@ _SIGS = {
     "plk_tree_optimize_all_branches": (_int, [_P, _int, _dbl, _dbl, _dbl, _int, _dp]),
     "plk_tree_full_traversal": (_int, [_P, _dp]),
     "plk_tree_counters": (_int, [_P, _i64p]),
+    "plk_tree_autotune": (_int, [_P, _int, ctypes.POINTER(_int), _dp]),
     "plk_tree_queue_branch": (_int, [_P, _int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp]),
     "plk_sim_create_on_tree": (_int, [ctypes.POINTER(_P), _int, _int, _ip, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
     # batched partitions
@@ -166,6 +169,13 @@ def _d(a):
 
 def device_count():
     return lib().plk_device_count()
+
+
+def probe_copy(nbytes=1 << 30, reps=5):
+    """Measured device-to-device copy bandwidth in GB/s (read + write bytes)."""
+    v = ctypes.c_double()
+    _check(lib().plk_probe_copy(nbytes, reps, ctypes.byref(v)))
+    return v.value
 
 
 class Problem:
@@ -292,6 +302,14 @@ class Tree:
         out = np.zeros(3, dtype=np.int64)
         _check(lib().plk_tree_counters(self.h, out.ctypes.data_as(_i64p)))
         return out
+
+    def autotune(self, reps=3):
+        w, ms = _int(), ctypes.c_double()
+        _check(lib().plk_tree_autotune(self.h, reps, ctypes.byref(w), ctypes.byref(ms)))
+        return w.value, ms.value
+
+    def set_tile_shape(self, wpc):
+        _check(lib().plk_set_tile_shape(self.context(), wpc))
 
     def set_timing(self, on):
         _check(lib().plk_set_timing(self.context(), int(on)))

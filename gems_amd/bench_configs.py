@@ -1,0 +1,259 @@
+"""Per-configuration measurements of the engine (the five workloads of the measurement
+plan), printed as one JSON document.
+
+  1  DNA GTR+G4, 32 taxa, 10,000 sites: full traversal + logL, all-branch df/ddf
+  2  20-state protein +G4, 100 taxa, 50,000 sites (also bench.py's headline)
+  3  300 unlinked partitions on 64 taxa (70% DNA, 30% protein, lognormal lengths)
+  4  61-state codon +G4, 48 taxa, 5,000 codons
+  5  serving: Poisson stream of logL / Newton requests, 16-512 taxa, 1k-100k sites
+
+Usage: python -m gems_amd.bench_configs [--configs 1,2,3,4,5] [--quick]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import gems_amd as G  # noqa: E402
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_oracle(p, reps=5, seconds=8.0, threads=None):
+    import support as S
+    threads = threads or max(1, min(16, os.cpu_count() or 1))
+    o = S.Oracle(p, safe=(p.n == 61), nthreads=threads)
+    a, b = 0, int(p.nei[0, 0])
+    o.lib.orc_clear_all(o.h)
+    ref = o.loglik(a, b)
+    ts = []
+    t0 = time.perf_counter()
+    while len(ts) < 2 or (len(ts) < reps and time.perf_counter() - t0 < seconds):
+        o.lib.orc_clear_all(o.h)
+        t1 = time.perf_counter()
+        o.loglik(a, b)
+        ts.append(time.perf_counter() - t1)
+    med = float(np.median(ts))
+    o.close()
+    return ref, (p.ntaxa - 2) * p.nptn / med, threads
+
+
+def traversal_rate(p, precision, steps, scaling=G.NORM):
+    g = G.Tree(p, precision=precision, scaling=scaling)
+    g.full_traversal()
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        v = g.full_traversal()
+    el = time.perf_counter() - t0
+    k_ms, k_n = g.timing(0)
+    g.close()
+    upd = (p.ntaxa - 2) * p.nptn
+    return {"updates_per_s": upd * steps / el, "ms_per_step": el / steps * 1e3,
+            "kernel_ms": k_ms / max(k_n, 1), "logl": v}
+
+
+def all_branch_derivs(p, precision, scaling=G.NORM):
+    """df/ddf on every branch (pre-order walk, partials recomputed as slots reorient)."""
+    g = G.Tree(p, precision=precision, scaling=scaling)
+    g.full_traversal()
+    br = [(u, int(p.nei[u, k])) for u in range(2 * p.ntaxa - 2) for k in range(3) if p.nei[u, k] > u]
+    t0 = time.perf_counter()
+    out = [g.derivs(u, v)[:2] for (u, v) in br]
+    el = time.perf_counter() - t0
+    c = g.counters()
+    g.close()
+    return br, np.array(out), el
+
+
+def config1(quick):
+    p = G.Problem(G.SEQ_DNA, 32, 10000, mean_len=0.1, alpha=0.5, seed=1)
+    log(f"config1: {p.nptn} patterns")
+    steps = 50 if quick else 400
+    r64 = traversal_rate(p, G.FP64, steps)
+    r32 = traversal_rate(p, G.FP32, steps)
+    ref, cpu, thr = cpu_oracle(p, reps=20)
+    import support as S
+    o = S.Oracle(p, nthreads=thr)
+    br, d64, t64 = all_branch_derivs(p, G.FP64)
+    _, d32, t32 = all_branch_derivs(p, G.FP32)
+    do = np.array([o.derivs(u, v)[:2] for (u, v) in br])
+    e64 = float(np.max(np.abs(d64 - do) / np.maximum(1.0, np.abs(do))))
+    e32 = float(np.max(np.abs(d32 - do) / np.maximum(1.0, np.abs(do))))
+    return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
+            "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
+            "all_branch_derivs_ms": {"fp64": t64 * 1e3, "fp32": t32 * 1e3, "branches": len(br)},
+            "deriv_max_rel_err": {"fp64": e64, "fp32": e32},
+            "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
+            "fp32_over_cpu": r32["updates_per_s"] / cpu}
+
+
+def config2(quick):
+    p = G.Problem(G.SEQ_PROTEIN, 100, 50000, mean_len=0.2, alpha=0.8, seed=1)
+    log(f"config2: {p.nptn} patterns")
+    steps = 5 if quick else 20
+    r64 = traversal_rate(p, G.FP64, steps)
+    r32 = traversal_rate(p, G.FP32, steps)
+    ref, cpu, thr = cpu_oracle(p, reps=5)
+    import support as S
+    o = S.Oracle(p, nthreads=thr)
+    br = [(0, int(p.nei[0, 0])), (100, int(p.nei[100, 0]))]
+    errs = {}
+    for prec, name in ((G.FP64, "fp64"), (G.FP32, "fp32")):
+        g = G.Tree(p, precision=prec)
+        e = 0.0
+        for (u, v) in br:
+            dg = g.derivs(u, v)[:2]
+            dd = o.derivs(u, v)[:2]
+            e = max(e, max(abs(dg[i] - dd[i]) / max(1.0, abs(dd[i])) for i in range(2)))
+        errs[name] = e
+        g.close()
+    return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
+            "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
+            "deriv_max_rel_err": errs, "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
+            "fp32_over_cpu": r32["updates_per_s"] / cpu}
+
+
+def config3(quick, nparts=300):
+    rng = np.random.default_rng(3)
+    base = G.Problem(G.SEQ_DNA, 64, 50, seed=3000)
+    probs = []
+    for i in range(nparts):
+        seq = G.SEQ_DNA if rng.random() < 0.7 else G.SEQ_PROTEIN
+        # lognormal, median 500 sites, sigma 1.0 (sigma is not fixed by the workload description)
+        nsites = int(np.clip(np.round(rng.lognormal(np.log(500), 1.0)), 100, 5000))
+        probs.append(G.Problem(seq, 64, nsites, mean_len=0.1 if seq == G.SEQ_DNA else 0.2,
+                               alpha=float(rng.uniform(0.3, 1.5)), seed=30000 + i, topology=base.nei))
+    upd = sum((p.ntaxa - 2) * p.nptn for p in probs)
+    log(f"config3: {nparts} partitions, {sum(p.nptn for p in probs)} patterns, {upd} updates per traversal")
+    br = [(0, int(p.nei[0, 0])) for p in probs]
+    out = {"partitions": nparts, "patterns": int(sum(p.nptn for p in probs)), "updates_per_traversal": upd,
+           "dna_partitions": int(sum(p.n == 4 for p in probs))}
+    steps = 5 if quick else 20
+    for prec, name in ((G.FP64, "fp64"), (G.FP32, "fp32")):
+        trees = [G.Tree(p, precision=prec) for p in probs]
+        # per-partition launches (the reference's loop over partition trees, one after another)
+        t0 = time.perf_counter()
+        for _ in range(max(1, steps // 5)):
+            seq_vals = [t.full_traversal() for t in trees]
+        t_seq = (time.perf_counter() - t0) / max(1, steps // 5)
+        b = G.Batch(trees)
+        grid, tune_ms = b.autotune(br, reps=2)
+        vals, _ = b.loglik(br, clear=True)
+        same = bool(np.array_equal(vals, np.array(seq_vals)))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            vals, tot = b.loglik(br, clear=True)
+        el = (time.perf_counter() - t0) / steps
+        ms, n, _ = b.timing()
+        out[name] = {"batched_ms": el * 1e3, "batched_updates_per_s": upd / el, "sequential_ms": t_seq * 1e3,
+                     "sequential_updates_per_s": upd / t_seq, "speedup_vs_sequential": t_seq / el,
+                     "autotuned_grid": grid, "work_items": b.items(), "batched_equals_alone": same,
+                     "total_logl": float(tot)}
+        b.close()
+        for t in trees:
+            t.close()
+    return out
+
+
+def config4(quick):
+    p = G.Problem(G.SEQ_CODON, 48, 5000, mean_len=0.1, alpha=0.5, seed=4)
+    log(f"config4: {p.nptn} patterns")
+    steps = 3 if quick else 10
+    r64 = traversal_rate(p, G.FP64, steps, scaling=G.SAFE)
+    r32 = traversal_rate(p, G.FP32, steps)
+    ref, cpu, thr = cpu_oracle(p, reps=3)
+    import support as S
+    o = S.Oracle(p, safe=True, nthreads=thr)
+    u, v = 0, int(p.nei[0, 0])
+    dd = o.derivs(u, v)[:2]
+    errs = {}
+    for prec, name, sc in ((G.FP64, "fp64", G.SAFE), (G.FP32, "fp32", G.NORM)):
+        g = G.Tree(p, precision=prec, scaling=sc)
+        dg = g.derivs(u, v)[:2]
+        errs[name] = max(abs(dg[i] - dd[i]) / max(1.0, abs(dd[i])) for i in range(2))
+        g.close()
+    flops = 4 * (4 * 61 * 61 + 61) * (p.ntaxa - 2) * p.nptn
+    return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
+            "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
+            "deriv_max_rel_err": errs,
+            "fp32_alg_tflops": flops / (r32["kernel_ms"] * 1e-3) / 1e12,
+            "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"]}
+
+
+def config5(quick, nreq=None):
+    rng = np.random.default_rng(5)
+    nreq = nreq or (16 if quick else 40)
+    specs = []
+    for i in range(nreq):
+        seq = G.SEQ_DNA if rng.random() < 0.5 else G.SEQ_PROTEIN
+        T = int(np.exp(rng.uniform(np.log(16), np.log(512))))
+        sites = int(np.exp(rng.uniform(np.log(1000), np.log(100000))))
+        kind = G.REQ_LOGL if rng.random() < 0.5 else G.REQ_OPT_BRANCH
+        specs.append((seq, T, sites, kind, 500 + i))
+    t0 = time.perf_counter()
+    probs = [G.Problem(s, T, n, mean_len=0.1 if s == G.SEQ_DNA else 0.2, alpha=0.5 if s == G.SEQ_DNA else 0.8, seed=sd)
+             for (s, T, n, k, sd) in specs]
+    log(f"config5: generated {nreq} requests in {time.perf_counter() - t0:.1f}s")
+    updates = sum((p.ntaxa - 2) * p.nptn for p in probs)
+    out = {"requests": nreq, "updates_per_pass": updates, "sweep": {}}
+    rates = [5.0, 20.0, 80.0, 1e9] if not quick else [20.0, 1e9]
+    for prec, name in ((G.FP64, "fp64"), (G.FP32, "fp32")):
+        res = []
+        for rate in rates:
+            reqs = [G.Request(p, kind=k, precision=prec, opt_branch=(T, int(p.nei[T, 0])))
+                    for p, (s, T, n, k, sd) in zip(probs, specs)]
+            gaps = rng.exponential(1.0 / rate, size=nreq)
+            arrivals = np.cumsum(gaps) - gaps[0]
+            srv = G.Server(nworkers=8, hbm_budget=200 << 30)
+            t0 = srv.now()
+            for r, at in zip(reqs, arrivals):
+                delay = t0 + at - srv.now()
+                if delay > 0:
+                    time.sleep(delay)
+                srv.submit(r)
+            srv.drain()
+            st = srv.stats()
+            srv.close()
+            res.append({"offered_rate": rate if rate < 1e8 else "saturation", "p50_s": st["p50_s"],
+                        "p99_s": st["p99_s"], "mean_s": st["mean_s"], "makespan_s": st["makespan_s"],
+                        "throughput_req_s": nreq / st["makespan_s"], "updates_per_s": updates / st["makespan_s"],
+                        "failed": st["failed"], "peak_concurrent": st["peak_concurrent"],
+                        "peak_hbm_gb": st["peak_hbm_bytes"] / 1e9})
+            log(f"  {name} rate {rate}: p50 {st['p50_s'] * 1e3:.1f} ms p99 {st['p99_s'] * 1e3:.1f} ms "
+                f"thr {nreq / st['makespan_s']:.1f} req/s")
+        out["sweep"][name] = res
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--configs", default="1,2,3,4,5")
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args()
+    fns = {"1": config1, "2": config2, "3": config3, "4": config4, "5": config5}
+    res = {}
+    for c in args.configs.split(","):
+        t0 = time.perf_counter()
+        res["config" + c] = fns[c](args.quick)
+        res["config" + c]["wall_s"] = time.perf_counter() - t0
+        log(json.dumps({"config" + c: res["config" + c]}))
+    txt = json.dumps(res, indent=1)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(txt)
+    print(txt)
+
+
+if __name__ == "__main__":
+    main()

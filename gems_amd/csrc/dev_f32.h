@@ -74,13 +74,21 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
 // Matrix-core traversal (20 and 61 states).  Workgroup = one 64-pattern tile, one
 // wavefront per category.  Lane l of a wave owns pattern column l%16 of each 16-pattern
 // block and rows 4*(l/16)..+3 of each 16-row x block of the MFMA output.
-template <int N, int MODE>
+//
+// Tile shape: WPC wavefronts per category split the tile's four 16-pattern blocks
+// (WPC = 1: one wave does all 64 patterns and reuses its A fragments four times;
+// WPC = 2 or 4: more waves in flight per tile).  Wave w handles category w / WPC and
+// pattern blocks [(w % WPC) * 4/WPC, ...).
+template <int N, int MODE, int WPC = 1>
 __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) {
     constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    constexpr int NPB = 4 / WPC;  // pattern blocks per wave
     const int lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
     const int ncat = P.ncat;
-    // one category per wave; waves beyond ncat (batched launches) only join barriers
-    const int cw = wave_id();
+    // waves beyond ncat * WPC (batched launches) only join barriers
+    const int wv = wave_id();
+    const int cw = wv / WPC;
+    const int pb0 = (wv % WPC) * NPB;
     const bool active = cw < ncat;
     const int c = active ? cw : 0;
     float *__restrict__ part = (float *)P.part;
@@ -91,53 +99,56 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     __shared__ float s_pi[64];
     if (MODE == MODE_LOGL && threadIdx.x < N) s_pi[threadIdx.x] = (float)P.pi[threadIdx.x];
 
+    // pattern blocks per renormalisation group: all four for <= 32 states; the 61-state
+    // model holds its 2 x 64 A fragments in registers, so it works on one block at a time
+    constexpr int PBG = (N > 32) ? 1 : NPB;
+    int sp = 0;  // double-buffered s_max
     for (int k = 0; k < P.nops; k++) {
         const DevOp op = P.ops[k];
         const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
-        floatx4 out[4][XB];
-        if (!active) {
-#pragma unroll
-            for (int pb = 0; pb < 4; pb++)
-#pragma unroll
-                for (int xb = 0; xb < XB; xb++) out[pb][xb] = floatx4{0, 0, 0, 0};
-        } else if (op.kind == KIND_TT) {
-            const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
-            const uint8_t *str = P.states + (int64_t)op.right * P.nptn_pad + (int64_t)tile * 64;
-            const float *tl0 = tabs;
-            const float *tr0 = tabs + (int64_t)(P.su + 1) * ncat * N;
-#pragma unroll
-            for (int pb = 0; pb < 4; pb++) {
-                const int sl = stl[pb * 16 + l16], sr = str[pb * 16 + l16];
-                const float *tl = tl0 + ((int64_t)sl * ncat + c) * N;
-                const float *tr = tr0 + ((int64_t)sr * ncat + c) * N;
-#pragma unroll
-                for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int x = xb * 16 + lg * 4 + r;
-                        out[pb][xb][r] = (x < N) ? tl[x] * tr[x] : 0.0f;
-                    }
-            }
-        } else {
-            // right child is internal: D_R = P_R . R on the matrix cores
+        const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
+        const uint8_t *str = P.states + (int64_t)op.right * P.nptn_pad + (int64_t)tile * 64;
+        const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
+        const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase;
+        float aL[XB][KS], aR[XB][KS];
+        if (active && op.kind != KIND_TT) {
             const float *__restrict__ FRm = M + (int64_t)(ncat + c) * FR;
-            float aR[XB][KS];
 #pragma unroll
             for (int xb = 0; xb < XB; xb++)
 #pragma unroll
                 for (int ks = 0; ks < KS; ks++) aR[xb][ks] = FRm[(xb * KS + ks) * 64 + lane];
-            const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
             if (op.kind == KIND_II) {
                 const float *__restrict__ FLm = M + (int64_t)c * FR;
-                float aL[XB][KS];
 #pragma unroll
                 for (int xb = 0; xb < XB; xb++)
 #pragma unroll
                     for (int ks = 0; ks < KS; ks++) aL[xb][ks] = FLm[(xb * KS + ks) * 64 + lane];
-                const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase;
+            }
+        }
+        float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride + cbase;
 #pragma unroll
-                for (int pb = 0; pb < 4; pb++) {
+        for (int g = pb0; g < pb0 + NPB; g += PBG) {
+            floatx4 out[PBG][XB];
+#pragma unroll
+            for (int j = 0; j < PBG; j++) {
+                const int pb = g + j;
+                if (!active) {
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++) out[j][xb] = floatx4{0, 0, 0, 0};
+                } else if (op.kind == KIND_TT) {
+                    const int sl = stl[pb * 16 + l16], sr = str[pb * 16 + l16];
+                    const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+                    const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int x = xb * 16 + lg * 4 + r;
+                            out[j][xb][r] = (x < N) ? tl[x] * tr[x] : 0.0f;
+                        }
+                } else {
+                    // D_R = P_R . R (and D_L = P_L . L) on the matrix cores
                     floatx4 dL[XB], dR[XB];
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++) {
@@ -147,79 +158,67 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
                     for (int ks = 0; ks < KS; ks++) {
                         const int y = ks * 4 + lg;
-                        const float bl = (y < N) ? L[y * 64 + pb * 16 + l16] : 0.0f;
-                        const float br = (y < N) ? R[y * 64 + pb * 16 + l16] : 0.0f;
-#pragma unroll
-                        for (int xb = 0; xb < XB; xb++) {
-                            dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[xb][ks], bl, dL[xb], 0, 0, 0);
-                            dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], br, dR[xb], 0, 0, 0);
-                        }
-                    }
-#pragma unroll
-                    for (int xb = 0; xb < XB; xb++) out[pb][xb] = dL[xb] * dR[xb];
-                }
-            } else {
-                const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
-#pragma unroll
-                for (int pb = 0; pb < 4; pb++) {
-                    floatx4 dR[XB];
-#pragma unroll
-                    for (int xb = 0; xb < XB; xb++) dR[xb] = floatx4{0, 0, 0, 0};
-#pragma unroll
-                    for (int ks = 0; ks < KS; ks++) {
-                        const int y = ks * 4 + lg;
                         const float br = (y < N) ? R[y * 64 + pb * 16 + l16] : 0.0f;
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++)
                             dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], br, dR[xb], 0, 0, 0);
-                    }
-                    const int sl = stl[pb * 16 + l16];
-                    const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+                        if (op.kind == KIND_II) {
+                            const float bl = (y < N) ? L[y * 64 + pb * 16 + l16] : 0.0f;
 #pragma unroll
-                    for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int x = xb * 16 + lg * 4 + r;
-                            out[pb][xb][r] = (x < N) ? tl[x] * dR[xb][r] : 0.0f;
+                            for (int xb = 0; xb < XB; xb++)
+                                dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[xb][ks], bl, dL[xb], 0, 0, 0);
                         }
+                    }
+                    if (op.kind == KIND_II) {
+#pragma unroll
+                        for (int xb = 0; xb < XB; xb++) out[j][xb] = dL[xb] * dR[xb];
+                    } else {
+                        const int sl = stl[pb * 16 + l16];
+                        const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+#pragma unroll
+                        for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                            for (int r = 0; r < 4; r++) {
+                                const int x = xb * 16 + lg * 4 + r;
+                                out[j][xb][r] = (x < N) ? tl[x] * dR[xb][r] : 0.0f;
+                            }
+                    }
+                }
+                // per-pattern maximum over this category (reduced over the 4 row groups)
+                float m = 0.0f;
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) m = fmaxf(m, fabsf(out[j][xb][r]));
+                m = fmaxf(m, __shfl_xor(m, 16, 64));
+                m = fmaxf(m, __shfl_xor(m, 32, 64));
+                if (lg == 0 && active) s_max[sp][c][pb * 16 + l16] = m;
+            }
+            __syncthreads();
+            // maximum over all categories -> power-of-two renormalisation and exponent sum
+#pragma unroll
+            for (int j = 0; j < PBG; j++) {
+                const int pl = (g + j) * 16 + l16;
+                float pm = 0.0f;
+                for (int cc = 0; cc < ncat; cc++) pm = fmaxf(pm, s_max[sp][cc][pl]);
+                const int e = norm_exponent(pm);
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int x = xb * 16 + lg * 4 + r;
+                        if (x < N && active) outp[x * 64 + pl] = ldexpf(out[j][xb][r], -e);
+                    }
+                if (cw == 0 && lg == 0) {  // one wave per pattern block keeps the exponents
+                    const int64_t p = (int64_t)tile * 64 + pl;
+                    int es = e;
+                    if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
+                    else if (op.kind == KIND_II)
+                        es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
+                    expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
                 }
             }
-        }
-        // per-pattern maximum over this category, then over all categories (LDS)
-#pragma unroll
-        for (int pb = 0; pb < 4; pb++) {
-            float m = 0.0f;
-#pragma unroll
-            for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) m = fmaxf(m, fabsf(out[pb][xb][r]));
-            m = fmaxf(m, __shfl_xor(m, 16, 64));
-            m = fmaxf(m, __shfl_xor(m, 32, 64));
-            if (lg == 0 && active) s_max[k & 1][c][pb * 16 + l16] = m;
-        }
-        __syncthreads();
-        float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride + cbase;
-#pragma unroll
-        for (int pb = 0; pb < 4; pb++) {
-            const int pl = pb * 16 + l16;
-            float pm = 0.0f;
-            for (int cc = 0; cc < ncat; cc++) pm = fmaxf(pm, s_max[k & 1][cc][pl]);
-            const int e = norm_exponent(pm);
-#pragma unroll
-            for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int x = xb * 16 + lg * 4 + r;
-                    if (x < N && active) outp[x * 64 + pl] = ldexpf(out[pb][xb][r], -e);
-                }
-            if (cw == 0 && lg == 0) {
-                const int64_t p = (int64_t)tile * 64 + pl;
-                int es = e;
-                if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
-                else if (op.kind == KIND_II)
-                    es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
-                expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
-            }
+            sp ^= 1;
         }
         // the next op reads these values from other lanes of this wavefront
         __syncthreads();
@@ -239,7 +238,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     const float *__restrict__ B = part + (int64_t)P.br_b * P.slot_stride + cbase;
     const uint8_t *sta = atip ? P.states + (int64_t)(-P.br_a - 1) * P.nptn_pad + (int64_t)tile * 64 : nullptr;
 #pragma unroll
-    for (int pb = 0; pb < 4; pb++) {
+    for (int pb = pb0; pb < pb0 + NPB; pb++) {
         floatx4 d[XB];
 #pragma unroll
         for (int xb = 0; xb < XB; xb++) d[xb] = floatx4{0, 0, 0, 0};
@@ -268,7 +267,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     }
     __syncthreads();
     double r0 = 0.0;
-    if (cw == 0) {
+    if (wv == 0) {
         const int64_t p = (int64_t)tile * 64 + lane;
         double lh = 0.0;
         for (int cc = 0; cc < ncat; cc++) lh += P.props[cc] * s_cat[cc][lane];

@@ -84,6 +84,7 @@ KParams make_params(Ctx &c) {
     p.su = c.su;
     p.safe = c.safe;
     p.ncw = std::min(c.ncat, 4);
+    p.wpc = c.wpc;
     p.part = c.d_part;
     p.slot_stride = (int64_t)c.nptn_pad * c.block;
     p.scale = c.d_scale;
@@ -228,6 +229,8 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     if (ntaxa < 3 || nptn < 1 || nslots < 1) return fail(PLK_ERR_ARG, "plk_create: need ntaxa >= 3, nptn >= 1, nslots >= 1");
     if (precision != PLK_FP64 && precision != PLK_FP32) return fail(PLK_ERR_ARG, "plk_create: bad precision form");
     if (scaling != PLK_SCALE_NORM && scaling != PLK_SCALE_SAFE) return fail(PLK_ERR_ARG, "plk_create: bad scaling");
+    if (precision == PLK_FP32 && nstates == 61 && ncat > 4)
+        return fail(PLK_ERR_ARG, "plk_create: the FP32 form of the 61-state model supports ncat <= 4");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(PLK_ERR_HIP, "plk_create: no HIP device visible (the engine has no CPU path)");
@@ -630,4 +633,48 @@ extern "C" int plk_get_exponents(plk_ctx *c, int slot, int32_t *exponents) {
     HIP_TRY(hipMemcpy(exponents, (char *)c->d_scale + (int64_t)4 * c->nptn_pad * slot, sizeof(int32_t) * c->nptn,
                       hipMemcpyDeviceToHost));
     return PLK_OK;
+}
+
+extern "C" int plk_set_tile_shape(plk_ctx *c, int waves_per_category) {
+    if (!c || (waves_per_category != 1 && waves_per_category != 2 && waves_per_category != 4))
+        return fail(PLK_ERR_ARG, "plk_set_tile_shape: waves per category must be 1, 2 or 4");
+    if (c->ncat * waves_per_category > 16) return fail(PLK_ERR_ARG, "plk_set_tile_shape: more than 16 waves per tile");
+    c->wpc = waves_per_category;
+    return PLK_OK;
+}
+
+// Device-to-device copy probe: measured HBM bandwidth (bytes read + written per second)
+// for roofline context, and a copy of known size for calibrating traffic counters.
+extern "C" int plk_probe_copy(int64_t bytes, int reps, double *gbps) {
+    clear_error();
+    if (bytes < 1024 || reps < 1 || !gbps) return fail(PLK_ERR_ARG, "plk_probe_copy: bad arguments");
+    hipStream_t s = nullptr;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc = PLK_OK;
+    float best = 1e30f;
+    if (hipMallocAsync(&a, bytes, s) != hipSuccess || hipMallocAsync(&b, bytes, s) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+        hipMemsetAsync(a, 1, bytes, s) != hipSuccess) {
+        rc = fail(PLK_ERR_HIP, "plk_probe_copy: allocation failed");
+    } else {
+        for (int r = 0; r <= reps && rc == PLK_OK; r++) {
+            float ms = 0.0f;
+            if (hipEventRecord(e0, s) != hipSuccess || hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+                hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                rc = fail(PLK_ERR_HIP, "plk_probe_copy: copy failed");
+            else if (r > 0)
+                best = std::min(best, ms);
+        }
+    }
+    if (a) (void)hipFreeAsync(a, s);
+    if (b) (void)hipFreeAsync(b, s);
+    (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    if (rc == PLK_OK) *gbps = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
+    return rc;
 }

@@ -42,6 +42,7 @@ enum : int { MODE_PARTIALS = 0, MODE_LOGL = 1, MODE_THETA = 2 };
 // Parameters of every likelihood kernel (passed by value).
 struct KParams {
     int n, ncat, block, nptn, nptn_pad, ntiles, su, safe, ncw;  // ncw = waves per tile
+    int wpc;                  // FP32 matrix-core tile shape: waves per category (1, 2, 4)
     void *part;               // slots x slot_stride reals
     int64_t slot_stride;      // reals per slot
     void *scale;              // slots x scale_stride counters (uint16 FP64 / int32 FP32)
@@ -96,6 +97,7 @@ struct Ctx {
     int theta_a = 0, theta_b = 0;
     bool theta_valid = false;
     bool timing = false;
+    int wpc = 1;  // FP32 matrix-core tile shape (plk_set_tile_shape)
     hipEvent_t ev[6] = {};
     double t_total[3] = {0, 0, 0};
     int64_t t_count[3] = {0, 0, 0};

@@ -10,9 +10,11 @@ __global__ void __launch_bounds__(256) k_pmatrix_f32(KParams P, const PJob *__re
     pmatrix_f32_body<N>(P, jobs[blockIdx.x]);
 }
 
-template <int N, int MODE>
-__global__ void __launch_bounds__(512) k_traverse_f32_mfma(KParams P) {
-    traverse_f32_mfma_body<N, MODE>(P, blockIdx.x);
+// WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1 (each wave
+// may use the whole 512-entry register file); WIDE allows 5-8 categories or WPC > 1.
+template <int N, int MODE, int WIDE, int WPC>
+__global__ void __launch_bounds__(WIDE ? 64 * 4 * 4 : 256) k_traverse_f32_mfma(KParams P) {
+    traverse_f32_mfma_body<N, MODE, WPC>(P, blockIdx.x);
 }
 
 template <int N, int MODE>
@@ -33,9 +35,28 @@ hipError_t traverse_f32_n(const KParams &p, int mode, hipStream_t s) {
         if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_valu<N, MODE_PARTIALS>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((k_traverse_f32_valu<N, MODE_LOGL>), grid, block, 0, s, p);
     } else {
-        dim3 block(64 * p.ncat);
-        if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS>), grid, block, 0, s, p);
-        else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL>), grid, block, 0, s, p);
+        // tile shape (waves per category) chosen by the caller's tuning; 1 unless the
+        // category count and register budget allow more
+        int wpc = p.wpc;
+        if (N > 32 || wpc < 1 || p.ncat * wpc > 16) wpc = 1;
+        dim3 block(64 * p.ncat * wpc);
+        if (wpc == 1 && p.ncat <= 4) {
+            if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 0, 1>), grid, block, 0, s, p);
+            else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 0, 1>), grid, block, 0, s, p);
+        } else if constexpr (N <= 32) {
+            if (wpc == 1) {
+                if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 1, 1>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 1, 1>), grid, block, 0, s, p);
+            } else if (wpc == 2) {
+                if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 1, 2>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 1, 2>), grid, block, 0, s, p);
+            } else {
+                if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 1, 4>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 1, 4>), grid, block, 0, s, p);
+            }
+        } else {
+            return hipErrorInvalidValue;
+        }
     }
     return hipGetLastError();
 }

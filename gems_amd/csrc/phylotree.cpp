@@ -449,3 +449,37 @@ extern "C" int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, 
     *len = t->length(dad, node);
     return PLK_OK;
 }
+
+// Tile-shape autotuner: time full traversals for each FP32 matrix-core tile shape
+// (waves per category) with HIP events and keep the fastest.  FP64 contexts have one
+// shape and return it.
+extern "C" int plk_tree_autotune(plk_tree *t, int reps, int *best_wpc, double *best_ms) {
+    if (!t || reps < 1) return fail(PLK_ERR_ARG, "plk_tree_autotune: bad arguments");
+    const int cands[3] = {1, 2, 4};
+    int best = 1;
+    double best_t = 1e300;
+    for (int wpc : cands) {
+        if (plk_set_tile_shape(t->ctx, wpc) != PLK_OK) continue;
+        double lnl = 0.0;
+        int rc = plk_tree_full_traversal(t, &lnl);  // warm-up
+        if (rc) return rc;
+        plk_set_timing(t->ctx, 1);
+        for (int r = 0; r < reps; r++) {
+            rc = plk_tree_full_traversal(t, &lnl);
+            if (rc) return rc;
+        }
+        double ms = 0.0;
+        int64_t n = 0;
+        plk_get_timing(t->ctx, 0, &ms, &n);
+        plk_set_timing(t->ctx, 0);
+        const double per = n ? ms / n : 1e300;
+        if (per < best_t) {
+            best_t = per;
+            best = wpc;
+        }
+    }
+    plk_set_tile_shape(t->ctx, best);
+    if (best_wpc) *best_wpc = best;
+    if (best_ms) *best_ms = best_t;
+    return PLK_OK;
+}

@@ -127,10 +127,20 @@ int plk_get_partial(plk_ctx *ctx, int slot, double *partial, uint16_t *scale);
  * stored value * 2^exponent), nptn int32. */
 int plk_get_exponents(plk_ctx *ctx, int slot, int32_t *exponents);
 
+/* FP32 form, 20-state model: wavefronts per category of a 64-pattern tile (1, 2 or 4).
+ * More waves per tile raise memory-level parallelism; fewer reuse each matrix fragment
+ * over more patterns.  Results do not depend on it. */
+int plk_set_tile_shape(plk_ctx *ctx, int waves_per_category);
+
 /* Kernel timing with HIP events recorded on the context's stream around every launch
  * of the partial-traversal kernel.  kind: 0 traversal, 1 P(t), 2 derivative. */
 int plk_set_timing(plk_ctx *ctx, int enable);
 int plk_get_timing(plk_ctx *ctx, int kind, double *total_ms, int64_t *launches);
+
+/* Measured device-to-device copy bandwidth ((read + written bytes)/s, best of reps) of a
+ * bytes-sized hipMemcpy: the attainable HBM rate, and a copy of known size for
+ * calibrating rocprof FETCH_SIZE/WRITE_SIZE. */
+int plk_probe_copy(int64_t bytes, int reps, double *gbps);
 
 /* Number of site-node partial updates (ops x patterns) executed since create. */
 int64_t plk_updates(plk_ctx *ctx);

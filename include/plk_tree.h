@@ -56,6 +56,9 @@ int plk_tree_full_traversal(plk_tree *t, double *logl);
  * evaluating it; returns the node references and length for plk_batch_loglik.  With
  * clear != 0 every partial is invalidated first (a full traversal). */
 int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, int *ref_a, int *ref_b, double *len);
+/* Measure every tile shape (plk_set_tile_shape) on full traversals of this tree and keep
+ * the fastest; returns it and its mean kernel time. */
+int plk_tree_autotune(plk_tree *t, int reps, int *best_wpc, double *best_ms);
 /* Counters: [0] partial updates (site x node), [1] derivative evaluations, [2] traversal ops */
 int plk_tree_counters(plk_tree *t, int64_t *out3);
 

@@ -128,3 +128,16 @@ def test_fp32_ragged():
         o = S.Oracle(pp)
         for (u, v) in branches(pp, 3):
             assert rel(g.loglik(u, v), o.loglik(u, v)) <= LOGL_TOL
+
+
+def test_fp32_tile_shapes_identical():
+    p = G.Problem(G.SEQ_PROTEIN, 30, 3000, mean_len=0.2, seed=12)
+    g = G.Tree(p, precision=G.FP32)
+    vals = []
+    for w in (1, 2, 4):
+        g.set_tile_shape(w)
+        vals.append(g.full_traversal())
+    assert vals[0] == vals[1] == vals[2], vals
+    best, ms = g.autotune(reps=2)
+    say("autotuned waves/category", best, "ms", ms)
+    assert g.full_traversal() == vals[0]
