@@ -281,6 +281,182 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 }
 
 // ---------------------------------------------------------------------------------------
+// Matrix-core traversal, pattern-block shape: each wavefront owns one 16-pattern block of
+// the tile across ALL categories (ncat <= 4).  The per-pattern maximum over categories is
+// then wave-local (registers + two lane shuffles), so wavefronts never wait for each
+// other: no LDS exchange and no barrier inside the op loop.  A wave re-reads its own
+// stores in other lanes; vector memory operations of one wavefront are performed in
+// order, and a wavefront-scope fence keeps the compiler from moving the next op's loads
+// above them.
+template <int N, int MODE>
+__device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    constexpr int NC = 4;
+    const int lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
+    const int ncat = P.ncat;
+    const int pb = wave_id();  // 16-pattern block of the tile
+    const bool active = pb < 4;
+    const int pbc = active ? pb : 0;
+    float *__restrict__ part = (float *)P.part;
+    int *__restrict__ expo = (int *)P.scale;
+    const int pl = pbc * 16 + l16;                       // pattern within the tile
+    const int64_t p = (int64_t)tile * 64 + pl;
+    auto cbase = [&](int c) { return ((int64_t)tile * ncat + c) * N * 64 + pbc * 16 + l16; };
+    __shared__ double s_wsum[16];
+
+    for (int k = 0; k < P.nops; k++) {
+        const DevOp op = P.ops[k];
+        const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
+        const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
+        int sl = 0, sr = 0;
+        if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
+        if (op.kind == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
+        floatx4 out[NC][XB];
+        float m = 0.0f;
+#pragma unroll
+        for (int c = 0; c < NC; c++) {
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++) out[c][xb] = floatx4{0, 0, 0, 0};
+            if (c >= ncat || !active) continue;
+            if (op.kind == KIND_TT) {
+                const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+                const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int x = xb * 16 + lg * 4 + r;
+                        out[c][xb][r] = (x < N) ? tl[x] * tr[x] : 0.0f;
+                    }
+            } else {
+                const float *__restrict__ FRm = M + (int64_t)(ncat + c) * FR;
+                const float *__restrict__ FLm = M + (int64_t)c * FR;
+                const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase(c);
+                const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase(c);
+                floatx4 dL[XB], dR[XB];
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++) {
+                    dL[xb] = floatx4{0, 0, 0, 0};
+                    dR[xb] = floatx4{0, 0, 0, 0};
+                }
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                    const int y = ks * 4 + lg;
+                    const float br = (y < N) ? R[y * 64] : 0.0f;
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++)
+                        dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(FRm[(xb * KS + ks) * 64 + lane], br, dR[xb], 0, 0, 0);
+                    if (op.kind == KIND_II) {
+                        const float bl = (y < N) ? L[y * 64] : 0.0f;
+#pragma unroll
+                        for (int xb = 0; xb < XB; xb++)
+                            dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(FLm[(xb * KS + ks) * 64 + lane], bl, dL[xb], 0, 0, 0);
+                    }
+                }
+                if (op.kind == KIND_II) {
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++) out[c][xb] = dL[xb] * dR[xb];
+                } else {
+                    const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int x = xb * 16 + lg * 4 + r;
+                            out[c][xb][r] = (x < N) ? tl[x] * dR[xb][r] : 0.0f;
+                        }
+                }
+            }
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) m = fmaxf(m, fabsf(out[c][xb][r]));
+        }
+        m = fmaxf(m, __shfl_xor(m, 16, 64));
+        m = fmaxf(m, __shfl_xor(m, 32, 64));
+        const int e = norm_exponent(m);
+        if (active) {
+            float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride;
+#pragma unroll
+            for (int c = 0; c < NC; c++) {
+                if (c >= ncat) continue;
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int x = xb * 16 + lg * 4 + r;
+                        if (x < N) outp[cbase(c) + x * 64] = ldexpf(out[c][xb][r], -e);
+                    }
+            }
+            if (lg == 0) {
+                int es = e;
+                if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
+                else if (op.kind == KIND_II)
+                    es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
+                expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (MODE == MODE_PARTIALS) return;
+
+    // -------- branch logL: lh = sum_c w_c sum_x pi_x A_c[x] (P_c(t) B_c)[x]
+    const float *__restrict__ Mt = (const float *)P.mats + (int64_t)P.nops * P.mat_stride;
+    const bool atip = P.br_a < 0;
+    const int sa = (atip && active) ? P.states[(int64_t)(-P.br_a - 1) * P.nptn_pad + p] : 0;
+    double lh = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+        if (c >= ncat || !active) continue;
+        const float *__restrict__ Ft = Mt + (int64_t)c * FR;
+        const float *__restrict__ A = part + (int64_t)(atip ? 0 : P.br_a) * P.slot_stride + cbase(c);
+        const float *__restrict__ B = part + (int64_t)P.br_b * P.slot_stride + cbase(c);
+        floatx4 d[XB];
+#pragma unroll
+        for (int xb = 0; xb < XB; xb++) d[xb] = floatx4{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int y = ks * 4 + lg;
+            const float b = (y < N) ? B[y * 64] : 0.0f;
+#pragma unroll
+            for (int xb = 0; xb < XB; xb++) d[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ft[(xb * KS + ks) * 64 + lane], b, d[xb], 0, 0, 0);
+        }
+        float acc = 0.0f;
+#pragma unroll
+        for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int x = xb * 16 + lg * 4 + r;
+                if (x < N) {
+                    const float a = atip ? (float)P.tipvec[sa * N + x] : A[x * 64];
+                    acc = fmaf((float)P.pi[x] * a, d[xb][r], acc);
+                }
+            }
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        lh += P.props[c] * (double)acc;
+    }
+    double contrib = 0.0;
+    if (active && lg == 0) {
+        lh = fabs(lh) + P.invar[p];
+        const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
+        const double l = log(lh) + e * kLn2;
+        if (p < P.nptn) {
+            P.ptn_lh[p] = l;
+            contrib = l * P.freq[p];
+        }
+    }
+    const double ws = wave_sum(contrib);
+    if (lane == 0) s_wsum[wave_id()] = ws;
+    __syncthreads();
+    double r0 = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < nwaves() && w < 16; w++) r0 += s_wsum[w];
+    finish_tiles(P, r0, 0.0, tile, P.ntiles);
+}
+
+// ---------------------------------------------------------------------------------------
 // Vector-ALU traversal for DNA: thread per (pattern, category), matrices in SGPRs.
 template <int N, int MODE>
 __device__ inline void traverse_f32_valu_body(const KParams &P, const int tile) {

@@ -636,8 +636,10 @@ extern "C" int plk_get_exponents(plk_ctx *c, int slot, int32_t *exponents) {
 }
 
 extern "C" int plk_set_tile_shape(plk_ctx *c, int waves_per_category) {
-    if (!c || (waves_per_category != 1 && waves_per_category != 2 && waves_per_category != 4))
-        return fail(PLK_ERR_ARG, "plk_set_tile_shape: waves per category must be 1, 2 or 4");
+    if (!c || (waves_per_category != 0 && waves_per_category != 1 && waves_per_category != 2 && waves_per_category != 4))
+        return fail(PLK_ERR_ARG, "plk_set_tile_shape: shape must be 0 (pattern-block waves) or 1, 2, 4 waves per category");
+    if (waves_per_category == 0 && c->ncat > 4)
+        return fail(PLK_ERR_ARG, "plk_set_tile_shape: pattern-block waves need ncat <= 4");
     if (c->ncat * waves_per_category > 16) return fail(PLK_ERR_ARG, "plk_set_tile_shape: more than 16 waves per tile");
     c->wpc = waves_per_category;
     return PLK_OK;

@@ -18,6 +18,11 @@ __global__ void __launch_bounds__(WIDE ? 64 * 4 * 4 : 256) k_traverse_f32_mfma(K
 }
 
 template <int N, int MODE>
+__global__ void __launch_bounds__(256) k_traverse_f32_pbw(KParams P) {
+    traverse_f32_pbw_body<N, MODE>(P, blockIdx.x);
+}
+
+template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_traverse_f32_valu(KParams P) {
     traverse_f32_valu_body<N, MODE>(P, blockIdx.x);
 }
@@ -35,8 +40,14 @@ hipError_t traverse_f32_n(const KParams &p, int mode, hipStream_t s) {
         if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_valu<N, MODE_PARTIALS>), grid, block, 0, s, p);
         else hipLaunchKernelGGL((k_traverse_f32_valu<N, MODE_LOGL>), grid, block, 0, s, p);
     } else {
-        // tile shape (waves per category) chosen by the caller's tuning; 1 unless the
-        // category count and register budget allow more
+        // tile shape chosen by the caller's tuning: 0 = one wave per 16-pattern block
+        // over all categories (ncat <= 4); 1, 2, 4 = waves per category
+        if (p.wpc == 0 && p.ncat <= 4) {
+            dim3 block(256);
+            if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_pbw<N, MODE_PARTIALS>), grid, block, 0, s, p);
+            else hipLaunchKernelGGL((k_traverse_f32_pbw<N, MODE_LOGL>), grid, block, 0, s, p);
+            return hipGetLastError();
+        }
         int wpc = p.wpc;
         if (N > 32 || wpc < 1 || p.ncat * wpc > 16) wpc = 1;
         dim3 block(64 * p.ncat * wpc);

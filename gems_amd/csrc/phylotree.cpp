@@ -455,7 +455,7 @@ extern "C" int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, 
 // shape and return it.
 extern "C" int plk_tree_autotune(plk_tree *t, int reps, int *best_wpc, double *best_ms) {
     if (!t || reps < 1) return fail(PLK_ERR_ARG, "plk_tree_autotune: bad arguments");
-    const int cands[3] = {1, 2, 4};
+    const int cands[4] = {0, 1, 2, 4};
     int best = 1;
     double best_t = 1e300;
     for (int wpc : cands) {

@@ -127,9 +127,11 @@ int plk_get_partial(plk_ctx *ctx, int slot, double *partial, uint16_t *scale);
  * stored value * 2^exponent), nptn int32. */
 int plk_get_exponents(plk_ctx *ctx, int slot, int32_t *exponents);
 
-/* FP32 form, 20-state model: wavefronts per category of a 64-pattern tile (1, 2 or 4).
- * More waves per tile raise memory-level parallelism; fewer reuse each matrix fragment
- * over more patterns.  Results do not depend on it. */
+/* FP32 form, 20/61-state models: tile shape of the matrix-core kernel.  0 = one
+ * wavefront per 16-pattern block over all categories (ncat <= 4; no cross-wave exchange);
+ * 1, 2 or 4 = wavefronts per category of a 64-pattern tile (more waves raise memory-level
+ * parallelism; fewer reuse each matrix fragment over more patterns).  Results do not
+ * depend on it. */
 int plk_set_tile_shape(plk_ctx *ctx, int waves_per_category);
 
 /* Kernel timing with HIP events recorded on the context's stream around every launch

@@ -134,10 +134,17 @@ def test_fp32_tile_shapes_identical():
     p = G.Problem(G.SEQ_PROTEIN, 30, 3000, mean_len=0.2, seed=12)
     g = G.Tree(p, precision=G.FP32)
     vals = []
-    for w in (1, 2, 4):
+    for w in (0, 1, 2, 4):
         g.set_tile_shape(w)
         vals.append(g.full_traversal())
-    assert vals[0] == vals[1] == vals[2], vals
+    assert vals[0] == vals[1] == vals[2] == vals[3], vals
     best, ms = g.autotune(reps=2)
     say("autotuned waves/category", best, "ms", ms)
     assert g.full_traversal() == vals[0]
+    # codon: pattern-block shape against the category-wave shape
+    pc = G.Problem(G.SEQ_CODON, 10, 400, seed=2)
+    gc = G.Tree(pc, precision=G.FP32)
+    gc.set_tile_shape(1)
+    a = gc.full_traversal()
+    gc.set_tile_shape(0)
+    assert gc.full_traversal() == a
