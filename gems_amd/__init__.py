@@ -112,6 +112,9 @@ _SIGS = {
     "plk_tree_optimize_all_branches": (_int, [_P, _int, _dbl, _dbl, _dbl, _int, _dp]),
     "plk_tree_full_traversal": (_int, [_P, _dp]),
     "plk_tree_counters": (_int, [_P, _i64p]),
+    "plk_tree_create_ex": (_int, [ctypes.POINTER(_P), _int, _int, _int, _int, _int, _int, _int]),
+    "plk_tree_all_branch_derivs": (_int, [_P, _ip, _ip, _dp, _dp]),
+    "plk_branch_derivs_multi": (_int, [_P, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp, _dp, _dp]),
     "plk_tree_autotune": (_int, [_P, _int, ctypes.POINTER(_int), _dp]),
     "plk_tree_queue_branch": (_int, [_P, _int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp]),
     "plk_sim_create_on_tree": (_int, [ctypes.POINTER(_P), _int, _int, _ip, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
@@ -220,11 +223,12 @@ class Problem:
 class Tree:
     """PhyloTreeGPU handle (plk_tree_*)."""
 
-    def __init__(self, prob, precision=FP64, scaling=NORM):
+    def __init__(self, prob, precision=FP64, scaling=NORM, all_directions=False):
         L = lib()
         self.p = prob
         h = _P()
-        _check(L.plk_tree_create(ctypes.byref(h), prob.n, len(prob.rates), prob.ntaxa, prob.nptn, precision, scaling))
+        _check(L.plk_tree_create_ex(ctypes.byref(h), prob.n, len(prob.rates), prob.ntaxa, prob.nptn, precision, scaling,
+                                    int(all_directions)))
         self.h = h
         self.set_model(prob)
         st = np.ascontiguousarray(prob.states, dtype=np.uint8)
@@ -302,6 +306,13 @@ class Tree:
         out = np.zeros(3, dtype=np.int64)
         _check(lib().plk_tree_counters(self.h, out.ctypes.data_as(_i64p)))
         return out
+
+    def all_branch_derivs(self):
+        nb = 2 * self.p.ntaxa - 3
+        a, b = np.zeros(nb, dtype=np.int32), np.zeros(nb, dtype=np.int32)
+        df, ddf = np.zeros(nb), np.zeros(nb)
+        _check(lib().plk_tree_all_branch_derivs(self.h, a.ctypes.data_as(_ip), b.ctypes.data_as(_ip), _d(df), _d(ddf)))
+        return list(zip(a.tolist(), b.tolist())), df, ddf
 
     def autotune(self, reps=3):
         w, ms = _int(), ctypes.c_double()

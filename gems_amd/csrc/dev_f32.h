@@ -628,7 +628,7 @@ __device__ inline void theta_f32_body(const KParams &P, const int tile) {
                 bi = fma(uinv[i * N + x], b[x], bi);
             }
             const double th = ai * bi;
-            theta[off + (int64_t)i * 64] = th;
+            if (theta) theta[off + (int64_t)i * 64] = th;  // null: derivatives only
             l0 = fma(s_val[0][c * N + i], th, l0);
             l1 = fma(s_val[1][c * N + i], th, l1);
             l2 = fma(s_val[2][c * N + i], th, l2);
@@ -642,7 +642,7 @@ __device__ inline void theta_f32_body(const KParams &P, const int tile) {
     if (w == 0) {
         const int *expo = (const int *)P.scale;
         const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
-        P.buf_scale[p] = e * kLn2;
+        if (P.buf_scale) P.buf_scale[p] = e * kLn2;
         double lh = 0.0, d1 = 0.0, d2 = 0.0;
         for (int c = 0; c < ncat; c++) {
             lh += s_cat[0][c][lane];

@@ -233,7 +233,7 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
             for (int i = 0; i < N; i++) {
                 const double a = atip ? tip[i] : A[off + i * 64];
                 const double th = a * B[off + i * 64];
-                theta[off + i * 64] = th;
+                if (theta) theta[off + i * 64] = th;  // null: derivatives only (many branches)
                 l0 = fma(v0[i], th, l0);
                 l1 = fma(v1[i], th, l1);
                 l2 = fma(v2[i], th, l2);
@@ -250,7 +250,7 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
     if (P.safe) {
         sc_min = s_sc[0][lane];
         for (int c = 1; c < ncat; c++) sc_min = min(sc_min, s_sc[c][lane]);
-        if (MODE == MODE_THETA) {
+        if (MODE == MODE_THETA && theta) {
             // rescale the stored theta of categories above the minimum (2146-2171, 2194-2225)
             for (int c = w; c < ncat; c += ncw) {
                 const int d = s_sc[c][lane] - sc_min;
@@ -296,7 +296,7 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
             r0 = wave_sum(p < P.nptn ? l * f : 0.0);
         } else {
             // buffer_scale_all (2178-2179, 2231-2232) and df/ddf (2504-2515)
-            P.buf_scale[p] = sc * kLogST;
+            if (P.buf_scale) P.buf_scale[p] = sc * kLogST;
             const double inv = 1.0 / lh;
             const double dfr = d1 * inv, ddfr = d2 * inv;
             const double t1 = dfr * f, t2 = ddfr * f;

@@ -321,7 +321,8 @@ extern "C" int plk_destroy(plk_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->d_part, c->d_scale, c->d_states, c->d_freq, c->d_invar, c->d_mats, c->d_model,
                     c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
-                    c->d_result, c->d_counter};
+                    c->d_result, c->d_counter, c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres,
+                    c->d_mcount};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, c->stream);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -679,4 +680,83 @@ extern "C" int plk_probe_copy(int64_t bytes, int reps, double *gbps) {
     (void)hipStreamDestroy(s);
     if (rc == PLK_OK) *gbps = 2.0 * (double)bytes / (best * 1e-3) / 1e9;
     return rc;
+}
+
+// computeLikelihoodDerv for many branches at once (the per-branch gradient_vector and
+// hessian_diagonal of tree/phylokernelnew.h:2567-2568): queued ops first, then one launch
+// over (branch, tile) items.  Theta is not kept (plk_loglik_from_theta needs a prior
+// plk_branch_derivs).
+extern "C" int plk_branch_derivs_multi(plk_ctx *c, int nbr, const int *a, const int *b, const double *t, double *df,
+                                       double *ddf) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nbr < 1 || !a || !b || !t || !df || !ddf) return fail(PLK_ERR_ARG, "plk_branch_derivs_multi: bad arguments");
+    std::vector<int> ra(a, a + nbr), rb(b, b + nbr);
+    for (int i = 0; i < nbr; i++) {
+        rc = branch_refs(*c, ra[i], rb[i]);
+        if (rc) return rc;
+        if (!std::isfinite(t[i]) || t[i] < 0) return fail(PLK_ERR_ARG, "plk_branch_derivs_multi: bad length");
+    }
+    rc = plk_flush(c);
+    if (rc) return rc;
+    if (nbr > c->mcap) {
+        void *old[] = {c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres, c->d_mcount};
+        for (void *q : old)
+            if (q) (void)hipFreeAsync(q, c->stream);
+        c->d_mparams = c->d_mitems = nullptr;
+        c->d_mtile = c->d_mres = nullptr;
+        c->d_mcount = nullptr;
+        c->mcap = 0;
+        const int cap = std::max(nbr, 2 * c->ntaxa);
+        HIP_TRY(hipMallocAsync(&c->d_mparams, sizeof(KParams) * cap, c->stream));
+        HIP_TRY(hipMallocAsync(&c->d_mitems, sizeof(BatchItem) * (size_t)cap * c->ntiles, c->stream));
+        HIP_TRY(hipMallocAsync((void **)&c->d_mtile, sizeof(double) * 2 * (size_t)cap * c->ntiles, c->stream));
+        HIP_TRY(hipMallocAsync((void **)&c->d_mres, sizeof(double) * 2 * cap, c->stream));
+        HIP_TRY(hipMallocAsync((void **)&c->d_mcount, sizeof(unsigned) * cap, c->stream));
+        HIP_TRY(hipMemsetAsync(c->d_mcount, 0, sizeof(unsigned) * cap, c->stream));
+        std::vector<BatchItem> items((size_t)cap * c->ntiles);
+        for (int i = 0; i < cap; i++)
+            for (int tt = 0; tt < c->ntiles; tt++) items[(size_t)i * c->ntiles + tt] = BatchItem{i, tt};
+        HIP_TRY(hipMemcpyAsync(c->d_mitems, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->mcap = cap;
+    }
+    KParams base = make_params(*c);
+    base.nops = 0;
+    base.theta = nullptr;
+    base.buf_scale = nullptr;
+    std::vector<KParams> ps(nbr, base);
+    for (int i = 0; i < nbr; i++) {
+        ps[i].br_a = ra[i];
+        ps[i].br_b = rb[i];
+        ps[i].br_len = t[i];
+        ps[i].tile_sums = c->d_mtile + (size_t)2 * c->ntiles * i;
+        ps[i].counter = c->d_mcount + i;
+        ps[i].result = c->d_mres + 2 * i;
+    }
+    rc = alloc_stage(*c, sizeof(KParams) * nbr + sizeof(double) * 2 * nbr);
+    if (rc) return rc;
+    std::memcpy(c->h_stage, ps.data(), sizeof(KParams) * nbr);
+    double *hres = (double *)((char *)c->h_stage + sizeof(KParams) * nbr);
+    HIP_TRY(hipMemcpyAsync(c->d_mparams, c->h_stage, sizeof(KParams) * nbr, hipMemcpyHostToDevice, c->stream));
+    timing_begin(*c, 2);
+    HIP_TRY(multi_derivs(c->prec, std::min(c->ncat, 4), (const KParams *)c->d_mparams, (const BatchItem *)c->d_mitems,
+                         nbr * c->ntiles, c->stream));
+    timing_end(*c, 2);
+    HIP_TRY(hipMemcpyAsync(hres, c->d_mres, sizeof(double) * 2 * nbr, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const bool used[3] = {false, false, true};
+    timing_collect(*c, used);
+    int status = PLK_OK;
+    for (int i = 0; i < nbr; i++) {
+        df[i] = hres[2 * i];
+        ddf[i] = hres[2 * i + 1];
+        if (!std::isfinite(df[i])) {
+            df[i] = ddf[i] = 0.0;
+            if (!c->safe) status = fail(PLK_ERR_NUMERIC, "Numerical underflow (lh-derivative): use SAFE scaling");
+        }
+    }
+    c->theta_valid = false;
+    return status;
 }

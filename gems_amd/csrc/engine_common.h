@@ -98,6 +98,11 @@ struct Ctx {
     bool theta_valid = false;
     bool timing = false;
     int wpc = 1;  // FP32 matrix-core tile shape (plk_set_tile_shape)
+    // many-branch derivative buffers (plk_branch_derivs_multi)
+    void *d_mparams = nullptr, *d_mitems = nullptr;
+    double *d_mtile = nullptr, *d_mres = nullptr;
+    unsigned *d_mcount = nullptr;
+    int mcap = 0;
     hipEvent_t ev[6] = {};
     double t_total[3] = {0, 0, 0};
     int64_t t_count[3] = {0, 0, 0};
@@ -132,4 +137,5 @@ struct BatchItem {
 hipError_t batch_pmatrix(int prec, const KParams *params, const PJob *jobs, int njobs, hipStream_t s);
 hipError_t batch_traverse(int prec, const KParams *params, const BatchItem *items, int nitems, int grid,
                           hipStream_t s);
+hipError_t multi_derivs(int prec, int ncw, const KParams *params, const BatchItem *items, int nitems, hipStream_t s);
 }  // namespace plk

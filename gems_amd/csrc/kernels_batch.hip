@@ -63,7 +63,39 @@ __global__ void __launch_bounds__(256) k_batch_traverse_f32(const KParams *param
     }
 }
 
+// df/ddf of many branches of one alignment in one launch: item = (branch, tile); each
+// branch has its own KParams (branch ends, length, reduction buffers) and theta is not
+// stored.  Same arithmetic as the single-branch kernels, so results are identical.
+__global__ void __launch_bounds__(256) k_multi_derivs_f64(const KParams *params, const BatchItem *items, int nitems) {
+    const BatchItem it = load_uniform(items + blockIdx.x);
+    const KParams P = load_uniform(params + it.part);
+    switch (P.n) {
+        case 4: traverse_f64_body<4, MODE_THETA>(P, it.tile); break;
+        case 20: traverse_f64_body<20, MODE_THETA>(P, it.tile); break;
+        case 61: traverse_f64_body<61, MODE_THETA>(P, it.tile); break;
+        default: break;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_multi_derivs_f32(const KParams *params, const BatchItem *items, int nitems) {
+    const BatchItem it = load_uniform(items + blockIdx.x);
+    const KParams P = load_uniform(params + it.part);
+    switch (P.n) {
+        case 4: theta_f32_body<4>(P, it.tile); break;
+        case 20: theta_f32_body<20>(P, it.tile); break;
+        case 61: theta_f32_body<61>(P, it.tile); break;
+        default: break;
+    }
+}
+
 }  // namespace
+
+hipError_t multi_derivs(int prec, int ncw, const KParams *params, const BatchItem *items, int nitems, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    if (prec == PLK_FP64) hipLaunchKernelGGL(k_multi_derivs_f64, dim3(nitems), dim3(64 * ncw), 0, s, params, items, nitems);
+    else hipLaunchKernelGGL(k_multi_derivs_f32, dim3(nitems), dim3(64 * ncw), 0, s, params, items, nitems);
+    return hipGetLastError();
+}
 
 hipError_t batch_pmatrix(int prec, const KParams *params, const PJob *jobs, int njobs, hipStream_t s) {
     if (njobs <= 0) return hipSuccess;

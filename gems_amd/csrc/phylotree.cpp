@@ -16,15 +16,17 @@ class PhyloTreeGPU {
     int ntaxa = 0, nnodes = 0;
     std::vector<int> nei;       // nnodes x 3
     std::vector<double> len;    // nnodes x 3
-    // LM_PER_NODE: internal node v owns slot v - ntaxa, oriented toward orient[v]
+    // LM_PER_NODE (default): internal node v owns slot v - ntaxa, oriented toward
+    // orient[v].  all_dirs: every direction (dad -> internal node) has its own slot
+    // 3*(node - ntaxa) + k, k = index of dad among node's neighbours (3(T-2) slots; what
+    // the many-branch derivative pass needs, and HBM is plentiful).
+    bool all_dirs = false;
     std::vector<int> orient;
-    std::vector<char> computed;
+    std::vector<char> computed;  // per node (LM_PER_NODE) or per slot (all_dirs)
     std::vector<plk_op> ops;
     int64_t n_derivs = 0, n_ops = 0;
 
     bool is_leaf(int v) const { return v < ntaxa; }
-    int slot(int v) const { return v - ntaxa; }
-    int ref(int v) const { return is_leaf(v) ? -(v + 1) : slot(v); }
 
     int dir(int a, int b) const {
         for (int k = 0; k < 3; k++)
@@ -32,6 +34,20 @@ class PhyloTreeGPU {
         return -1;
     }
     double length(int a, int b) const { return len[dir(a, b)]; }
+    // slot of the partial of node's subtree seen from dad (node internal)
+    int slot_of(int dad, int node) const { return all_dirs ? 3 * (node - ntaxa) + (dir(node, dad) - node * 3) : node - ntaxa; }
+    // node reference of the partial of `to`'s subtree seen from `from`
+    int ref_dir(int from, int to) const { return is_leaf(to) ? -(to + 1) : slot_of(from, to); }
+    bool is_computed(int dad, int node) const {
+        return all_dirs ? computed[slot_of(dad, node)] != 0 : (orient[node] == dad && computed[node]);
+    }
+    void set_computed(int dad, int node, bool v) {
+        if (all_dirs) computed[slot_of(dad, node)] = v;
+        else {
+            orient[node] = dad;
+            computed[node] = v;
+        }
+    }
 
     int subtree_size(int node, int dad) const {
         if (is_leaf(node)) return 1;
@@ -48,9 +64,8 @@ class PhyloTreeGPU {
     // decreasing subtree size; reorientPartialLh (6011-6025) moves the node's slot.
     void collect(int dad, int node) {
         if (is_leaf(node)) return;
-        if (orient[node] == dad && computed[node]) return;
-        orient[node] = dad;
-        computed[node] = 0;
+        if (is_computed(dad, node)) return;
+        set_computed(dad, node, false);
         int kids[2], nk = 0;
         for (int k = 0; k < 3; k++) {
             int x = nei[node * 3 + k];
@@ -63,11 +78,11 @@ class PhyloTreeGPU {
             collect(node, kids[1]);
             collect(node, kids[0]);
         }
-        computed[node] = 1;
+        set_computed(dad, node, true);
         plk_op op{};
-        op.out_slot = slot(node);
-        op.child[0] = ref(kids[0]);
-        op.child[1] = ref(kids[1]);
+        op.out_slot = slot_of(dad, node);
+        op.child[0] = ref_dir(node, kids[0]);
+        op.child[1] = ref_dir(node, kids[1]);
         op.len[0] = length(node, kids[0]);
         op.len[1] = length(node, kids[1]);
         ops.push_back(op);
@@ -88,7 +103,10 @@ class PhyloTreeGPU {
         for (int k = 0; k < 3; k++) {
             int x = nei[node * 3 + k];
             if (x < 0 || x == dad) continue;
-            if (!is_leaf(node) && orient[node] == x) computed[node] = 0;
+            if (!is_leaf(node)) {
+                if (all_dirs) computed[slot_of(x, node)] = 0;
+                else if (orient[node] == x) computed[node] = 0;
+            }
             clear_reverse(x, node);
         }
     }
@@ -108,7 +126,7 @@ class PhyloTreeGPU {
         if (dir(dad, node) < 0) return fail(PLK_ERR_ARG, "no such branch");
         int rc = plan(dad, node);
         if (rc) return rc;
-        return plk_branch_loglik(ctx, ref(dad), ref(node), length(dad, node), logl, ptn_lh);
+        return plk_branch_loglik(ctx, ref_dir(node, dad), ref_dir(dad, node), length(dad, node), logl, ptn_lh);
     }
 
     int derivs(int dad, int node, int reuse, double *df, double *ddf) {
@@ -116,7 +134,51 @@ class PhyloTreeGPU {
         int rc = plan(dad, node);
         if (rc) return rc;
         n_derivs++;
-        return plk_branch_derivs(ctx, ref(dad), ref(node), length(dad, node), reuse, df, ddf);
+        return plk_branch_derivs(ctx, ref_dir(node, dad), ref_dir(dad, node), length(dad, node), reuse, df, ddf);
+    }
+
+    // df/ddf of every branch at its current length (IQ-TREE fills gradient_vector and
+    // hessian_diagonal one computeLikelihoodDerv call at a time).  With all_dirs every
+    // missing partial is computed in one traversal launch and all branches in one
+    // derivative launch; LM_PER_NODE evaluates the branches one after another.
+    int all_branch_derivs(std::vector<int> &ba, std::vector<int> &bb, std::vector<double> &df, std::vector<double> &ddf) {
+        ba.clear();
+        bb.clear();
+        for (int u = 0; u < nnodes; u++)
+            for (int k = 0; k < 3; k++) {
+                int v = nei[u * 3 + k];
+                if (v > u) {
+                    ba.push_back(u);
+                    bb.push_back(v);
+                }
+            }
+        const int nb = (int)ba.size();
+        df.assign(nb, 0.0);
+        ddf.assign(nb, 0.0);
+        if (!all_dirs) {
+            for (int i = 0; i < nb; i++) {
+                int rc = derivs(ba[i], bb[i], 0, &df[i], &ddf[i]);
+                if (rc) return rc;
+            }
+            return PLK_OK;
+        }
+        ops.clear();
+        std::vector<int> ra(nb), rb(nb);
+        std::vector<double> t(nb);
+        for (int i = 0; i < nb; i++) {
+            collect(bb[i], ba[i]);
+            collect(ba[i], bb[i]);
+            ra[i] = ref_dir(bb[i], ba[i]);
+            rb[i] = ref_dir(ba[i], bb[i]);
+            t[i] = length(ba[i], bb[i]);
+        }
+        n_ops += (int64_t)ops.size();
+        if (!ops.empty()) {
+            int rc = plk_update_partials(ctx, ops.data(), (int)ops.size());
+            if (rc) return rc;
+        }
+        n_derivs += nb;
+        return plk_branch_derivs_multi(ctx, nb, ra.data(), rb.data(), t.data(), df.data(), ddf.data());
     }
 
     // computeFuncDerv (tree/phylotree.cpp:2555-2562): derivatives of -logL
@@ -289,24 +351,31 @@ struct plk_tree : public plk::PhyloTreeGPU {};
 
 using plk::fail;
 
-extern "C" int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling) {
+extern "C" int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision,
+                                  int scaling, int all_directions) {
     if (!out) return fail(PLK_ERR_ARG, "plk_tree_create: null out");
     *out = nullptr;
     if (ntaxa < 3) return fail(PLK_ERR_ARG, "plk_tree_create: need at least 3 taxa");
     auto *t = new plk_tree();
     t->ntaxa = ntaxa;
     t->nnodes = 2 * ntaxa - 2;
+    t->all_dirs = all_directions != 0;
     t->nei.assign((size_t)t->nnodes * 3, -1);
     t->len.assign((size_t)t->nnodes * 3, 0.0);
     t->orient.assign(t->nnodes, -1);
-    t->computed.assign(t->nnodes, 0);
-    int rc = plk_create(&t->ctx, nstates, ncat, ntaxa, nptn, ntaxa - 2, precision, scaling);
+    t->computed.assign(t->all_dirs ? (size_t)3 * (ntaxa - 2) : (size_t)t->nnodes, 0);
+    const int nslots = t->all_dirs ? 3 * (ntaxa - 2) : ntaxa - 2;
+    int rc = plk_create(&t->ctx, nstates, ncat, ntaxa, nptn, nslots, precision, scaling);
     if (rc) {
         delete t;
         return rc;
     }
     *out = t;
     return PLK_OK;
+}
+
+extern "C" int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling) {
+    return plk_tree_create_ex(out, nstates, ncat, ntaxa, nptn, precision, scaling, 0);
 }
 
 extern "C" int plk_tree_destroy(plk_tree *t) {
@@ -410,7 +479,7 @@ extern "C" int plk_tree_get_partial(plk_tree *t, int dad, int node, double *part
         int rc = plk_update_partials(t->ctx, t->ops.data(), (int)t->ops.size());
         if (rc) return rc;
     }
-    return plk_get_partial(t->ctx, t->slot(node), partial, scale);
+    return plk_get_partial(t->ctx, t->slot_of(dad, node), partial, scale);
 }
 
 extern "C" int plk_tree_optimize_one_branch(plk_tree *t, int n1, int n2, double min_bl, double max_bl, int max_nr,
@@ -444,8 +513,8 @@ extern "C" int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, 
     if (clear) t->clear_all();
     int rc = t->plan(dad, node);
     if (rc) return rc;
-    *ref_a = t->ref(dad);
-    *ref_b = t->ref(node);
+    *ref_a = t->ref_dir(node, dad);
+    *ref_b = t->ref_dir(dad, node);
     *len = t->length(dad, node);
     return PLK_OK;
 }
@@ -482,4 +551,18 @@ extern "C" int plk_tree_autotune(plk_tree *t, int reps, int *best_wpc, double *b
     if (best_wpc) *best_wpc = best;
     if (best_ms) *best_ms = best_t;
     return PLK_OK;
+}
+
+extern "C" int plk_tree_all_branch_derivs(plk_tree *t, int32_t *branch_a, int32_t *branch_b, double *df, double *ddf) {
+    if (!t || !df || !ddf) return fail(PLK_ERR_ARG, "plk_tree_all_branch_derivs: bad arguments");
+    std::vector<int> a, b;
+    std::vector<double> d1, d2;
+    int rc = t->all_branch_derivs(a, b, d1, d2);
+    for (size_t i = 0; i < a.size(); i++) {
+        if (branch_a) branch_a[i] = a[i];
+        if (branch_b) branch_b[i] = b[i];
+        df[i] = d1[i];
+        ddf[i] = d2[i];
+    }
+    return rc;
 }

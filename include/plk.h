@@ -114,6 +114,12 @@ int plk_branch_loglik(plk_ctx *ctx, int a, int b, double t, double *logl, double
 int plk_branch_derivs(plk_ctx *ctx, int a, int b, double t, int reuse_theta, double *df,
                       double *ddf);
 
+/* computeLikelihoodDerv for nbr branches in one launch (every branch's gradient_vector /
+ * hessian_diagonal entry, tree/phylokernelnew.h:2567-2568): a[i], b[i] node references,
+ * t[i] lengths.  The theta buffer is not kept. */
+int plk_branch_derivs_multi(plk_ctx *ctx, int nbr, const int *a, const int *b, const double *t, double *df,
+                            double *ddf);
+
 /* computeLikelihoodFromBuffer (tree/phylokernelnew.h:3284-3450): logL at length t from the
  * theta buffer of the last plk_branch_derivs call. */
 int plk_loglik_from_theta(plk_ctx *ctx, double t, double *logl);

@@ -27,6 +27,10 @@ extern "C" {
 typedef struct plk_tree plk_tree;
 
 int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling);
+/* all_directions != 0: one slot per direction (3(T-2) slots) instead of LM_PER_NODE's one
+ * per internal node, so every branch's two partials coexist (plk_tree_all_branch_derivs). */
+int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling,
+                       int all_directions);
 int plk_tree_destroy(plk_tree *t);
 plk_ctx *plk_tree_context(plk_tree *t);
 int plk_tree_set_model(plk_tree *t, const double *eval, const double *evec, const double *inv_evec,
@@ -59,6 +63,9 @@ int plk_tree_queue_branch(plk_tree *t, int dad, int node, int clear, int *ref_a,
 /* Measure every tile shape (plk_set_tile_shape) on full traversals of this tree and keep
  * the fastest; returns it and its mean kernel time. */
 int plk_tree_autotune(plk_tree *t, int reps, int *best_wpc, double *best_ms);
+/* df/ddf of all 2T-3 branches at their current lengths (branches listed as (a, b), a < b,
+ * in node order); one traversal launch + one derivative launch in all-directions mode. */
+int plk_tree_all_branch_derivs(plk_tree *t, int32_t *branch_a, int32_t *branch_b, double *df, double *ddf);
 /* Counters: [0] partial updates (site x node), [1] derivative evaluations, [2] traversal ops */
 int plk_tree_counters(plk_tree *t, int64_t *out3);
 

@@ -169,3 +169,32 @@ def test_invalid_ops_rejected():
     # the context still works afterwards
     o = S.Oracle(p)
     assert rel(g.loglik(0, int(p.nei[0, 0])), o.loglik(0, int(p.nei[0, 0]))) <= TOL
+
+
+@pytest.mark.parametrize("prec", [G.FP64, G.FP32], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("safe", [0, 1])
+def test_all_branch_derivs(prec, safe):
+    if prec == G.FP32 and safe:
+        pytest.skip("the FP32 form has no SAFE mode")
+    p = G.Problem(G.SEQ_DNA, 20, 2000, seed=13)
+    say("all-branch derivs", prec, safe)
+    g = G.Tree(p, precision=prec, scaling=safe, all_directions=True)
+    g.full_traversal()
+    br, df, ddf = g.all_branch_derivs()
+    assert len(br) == 2 * p.ntaxa - 3
+    single = G.Tree(p, precision=prec, scaling=safe)
+    o = S.Oracle(p, safe=bool(safe))
+    tol = 1e-9 if prec == G.FP64 else 1e-3
+    for (u, v), d1, d2 in zip(br, df, ddf):
+        s1, s2, _ = single.derivs(u, v)
+        assert d1 == s1 and d2 == s2, (u, v)
+        o1, o2, _ = o.derivs(u, v)
+        assert abs(d1 - o1) <= tol * max(1.0, abs(o1)) and abs(d2 - o2) <= tol * max(1.0, abs(o2))
+    # after changing a length the pass recomputes only what depends on it
+    u, v = br[5]
+    g.set_length(u, v, 0.3)
+    single.set_length(u, v, 0.3)
+    br2, df2, _ = g.all_branch_derivs()
+    for (a, b), d1 in zip(br2, df2):
+        assert d1 == single.derivs(a, b)[0]
+    assert g.loglik(*br[0]) == single.loglik(*br[0])
