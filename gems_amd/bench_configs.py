@@ -62,17 +62,19 @@ def traversal_rate(p, precision, steps, scaling=G.NORM):
             "kernel_ms": k_ms / max(k_n, 1), "logl": v}
 
 
-def all_branch_derivs(p, precision, scaling=G.NORM):
-    """df/ddf on every branch (pre-order walk, partials recomputed as slots reorient)."""
-    g = G.Tree(p, precision=precision, scaling=scaling)
-    g.full_traversal()
-    br = [(u, int(p.nei[u, k])) for u in range(2 * p.ntaxa - 2) for k in range(3) if p.nei[u, k] > u]
+def all_branch_derivs(p, precision, scaling=G.NORM, reps=20):
+    """df/ddf on every branch from scratch: all 3(T-2) directional partials (one traversal
+    launch, all-directions slots) + every branch's derivatives (one launch)."""
+    g = G.Tree(p, precision=precision, scaling=scaling, all_directions=True)
+    lib = G.lib()
+    g.all_branch_derivs()
     t0 = time.perf_counter()
-    out = [g.derivs(u, v)[:2] for (u, v) in br]
-    el = time.perf_counter() - t0
-    c = g.counters()
+    for _ in range(reps):
+        lib.plk_tree_clear_all(g.h)
+        br, df, ddf = g.all_branch_derivs()
+    el = (time.perf_counter() - t0) / reps
     g.close()
-    return br, np.array(out), el
+    return br, np.stack([df, ddf], axis=1), el
 
 
 def config1(quick):
@@ -91,7 +93,10 @@ def config1(quick):
     e32 = float(np.max(np.abs(d32 - do) / np.maximum(1.0, np.abs(do))))
     return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
             "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
-            "all_branch_derivs_ms": {"fp64": t64 * 1e3, "fp32": t32 * 1e3, "branches": len(br)},
+            "all_branch_derivs_ms": {"fp64": t64 * 1e3, "fp32": t32 * 1e3, "branches": len(br),
+                                     "updates_per_pass": 3 * (p.ntaxa - 2) * p.nptn,
+                                     "fp64_updates_per_s": 3 * (p.ntaxa - 2) * p.nptn / t64,
+                                     "fp32_updates_per_s": 3 * (p.ntaxa - 2) * p.nptn / t32},
             "deriv_max_rel_err": {"fp64": e64, "fp32": e32},
             "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
             "fp32_over_cpu": r32["updates_per_s"] / cpu}
