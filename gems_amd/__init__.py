@@ -74,6 +74,13 @@ _SIGS = {
     "plk_branch_loglik": (_int, [_P, _int, _int, _dbl, _dp, _dp]),
     "plk_branch_derivs": (_int, [_P, _int, _int, _dbl, _int, _dp, _dp]),
     "plk_loglik_from_theta": (_int, [_P, _dbl, _dp]),
+    "plk_newton_branch": (_int, [_P, _int, _int, _dbl, _dbl, _dbl, _dbl, _int, _dp, _ip]),
+    # UFBoot RELL
+    "plk_rell_create": (_int, [ctypes.POINTER(_P), _int, _int, ctypes.POINTER(ctypes.c_float)]),
+    "plk_rell_destroy": (_int, [_P]),
+    "plk_rell_eval": (_int, [_P, _P, _dp]),
+    "plk_rell_save_tree": (_int, [_P, _P, _dbl, _dbl, _dp, _ip, ctypes.POINTER(_int)]),
+    "plk_rell_get": (_int, [_P, _dp, _ip, _dp]),
     "plk_get_partial": (_int, [_P, _int, _dp, _u16p]),
     "plk_set_timing": (_int, [_P, _int]),
     "plk_get_timing": (_int, [_P, _int, _dp, _i64p]),
@@ -109,6 +116,7 @@ _SIGS = {
     "plk_tree_loglik_from_theta": (_int, [_P, _int, _int, _dp]),
     "plk_tree_get_partial": (_int, [_P, _int, _int, _dp, _u16p]),
     "plk_tree_optimize_one_branch": (_int, [_P, _int, _int, _dbl, _dbl, _int, _dp]),
+    "plk_tree_set_newton": (_int, [_P, _int]),
     "plk_tree_optimize_all_branches": (_int, [_P, _int, _dbl, _dbl, _dbl, _int, _dp]),
     "plk_tree_full_traversal": (_int, [_P, _dp]),
     "plk_tree_counters": (_int, [_P, _i64p]),
@@ -297,6 +305,10 @@ class Tree:
         _check(lib().plk_tree_optimize_one_branch(self.h, a, b, min_bl, max_bl, max_nr, ctypes.byref(v)))
         return v.value
 
+    def set_newton(self, device_resident):
+        """Newton solves on the device (True, default) or as a host loop (False)."""
+        _check(lib().plk_tree_set_newton(self.h, 1 if device_resident else 0))
+
     def optimize_all(self, iterations=100, tol=1e-3, min_bl=1e-6, max_bl=10.0, max_nr=100):
         v = ctypes.c_double()
         _check(lib().plk_tree_optimize_all_branches(self.h, iterations, tol, min_bl, max_bl, max_nr, ctypes.byref(v)))
@@ -458,3 +470,43 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+class Rell:
+    """UFBoot replicates resident on the device (plk_rell_*): counts is nsamples x nptn."""
+
+    def __init__(self, counts):
+        c = np.ascontiguousarray(counts, dtype=np.float32)
+        self.ns, self.nptn = c.shape
+        self.h = _P()
+        _check(lib().plk_rell_create(ctypes.byref(self.h), self.ns, self.nptn,
+                                     c.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def eval(self, ctx):
+        out = np.zeros(self.ns)
+        _check(lib().plk_rell_eval(self.h, ctx, _d(out)))
+        return out
+
+    def save_tree(self, ctx, cur_logl, uniform, eps=0.5):
+        u = np.ascontiguousarray(uniform, dtype=np.float64)
+        imp = np.zeros(self.ns, dtype=np.int32)
+        n = _int()
+        _check(lib().plk_rell_save_tree(self.h, ctx, cur_logl, eps, _d(u), imp.ctypes.data_as(_ip), ctypes.byref(n)))
+        return imp, n.value
+
+    def state(self):
+        lo, orig = np.zeros(self.ns), np.zeros(self.ns)
+        cnt = np.zeros(self.ns, dtype=np.int32)
+        _check(lib().plk_rell_get(self.h, _d(lo), cnt.ctypes.data_as(_ip), _d(orig)))
+        return lo, cnt, orig
+
+    def close(self):
+        if self.h:
+            lib().plk_rell_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -25,6 +25,15 @@ typedef const __attribute__((address_space(4))) float *cfptr;
 __device__ __forceinline__ cdptr as_const(const double *p) { return (cdptr)p; }
 __device__ __forceinline__ cfptr as_constf(const float *p) { return (cfptr)p; }
 
+// The same pointer, hidden from loop-invariant code motion: a matrix read through it
+// inside a loop is re-read (scalar-cache hits) each iteration instead of being hoisted
+// out and parked in VGPR lanes, which would cost a v_readlane per use.
+template <typename T>
+__device__ __forceinline__ T *opaque(T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 __device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
 
@@ -44,6 +53,15 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
+// f(integral_constant<kind>) for a run-time op kind (uniform over a launch), so each
+// kind compiles to its own branch-free body.
+template <typename F>
+__device__ __forceinline__ void with_kind(int kind, F &&f) {
+    if (kind == KIND_II) f(std::integral_constant<int, KIND_II>{});
+    else if (kind == KIND_TI) f(std::integral_constant<int, KIND_TI>{});
+    else f(std::integral_constant<int, KIND_TT>{});
+}
+
 // x-loop over output states: unrolled for small state counts, a plain loop for the
 // 61-state codon model.
 template <int N, typename F>
@@ -60,7 +78,8 @@ __device__ __forceinline__ void for_states(F &&f) {
 // last workgroup to finish a tile of this alignment sums the tile values in index
 // order, so the result does not depend on scheduling (or on other alignments sharing
 // the launch); it then re-arms the counter for the next launch (no memset, graph-safe).
-__device__ inline void finish_tiles(const KParams &P, double v0, double v1, int tile, int ntiles) {
+// Returns true in the workgroup that wrote the result.
+__device__ inline bool finish_tiles(const KParams &P, double v0, double v1, int tile, int ntiles) {
     __shared__ int s_last;
     if (threadIdx.x == 0) {
         P.tile_sums[2 * tile] = v0;
@@ -88,6 +107,7 @@ __device__ inline void finish_tiles(const KParams &P, double v0, double v1, int 
         }
     }
     __syncthreads();
+    return s_last;
 }
 
 }  // namespace dev

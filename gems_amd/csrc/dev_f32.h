@@ -105,6 +105,10 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     int sp = 0;  // double-buffered s_max
     for (int k = 0; k < P.nops; k++) {
         const DevOp op = P.ops[k];
+        // the op kind is uniform over the launch: each kind gets its own straight-line
+        // code, so all loads of an op are issued together
+        with_kind(op.kind, [&](auto KC) {
+        constexpr int KIND = decltype(KC)::value;
         const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
@@ -112,13 +116,13 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
         const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
         const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase;
         float aL[XB][KS], aR[XB][KS];
-        if (active && op.kind != KIND_TT) {
+        if (active && KIND != KIND_TT) {
             const float *__restrict__ FRm = M + (int64_t)(ncat + c) * FR;
 #pragma unroll
             for (int xb = 0; xb < XB; xb++)
 #pragma unroll
                 for (int ks = 0; ks < KS; ks++) aR[xb][ks] = FRm[(xb * KS + ks) * 64 + lane];
-            if (op.kind == KIND_II) {
+            if (KIND == KIND_II) {
                 const float *__restrict__ FLm = M + (int64_t)c * FR;
 #pragma unroll
                 for (int xb = 0; xb < XB; xb++)
@@ -136,7 +140,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 if (!active) {
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++) out[j][xb] = floatx4{0, 0, 0, 0};
-                } else if (op.kind == KIND_TT) {
+                } else if (KIND == KIND_TT) {
                     const int sl = stl[pb * 16 + l16], sr = str[pb * 16 + l16];
                     const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
                     const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
@@ -162,14 +166,14 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++)
                             dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], br, dR[xb], 0, 0, 0);
-                        if (op.kind == KIND_II) {
+                        if (KIND == KIND_II) {
                             const float bl = (y < N) ? L[y * 64 + pb * 16 + l16] : 0.0f;
 #pragma unroll
                             for (int xb = 0; xb < XB; xb++)
                                 dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[xb][ks], bl, dL[xb], 0, 0, 0);
                         }
                     }
-                    if (op.kind == KIND_II) {
+                    if (KIND == KIND_II) {
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++) out[j][xb] = dL[xb] * dR[xb];
                     } else {
@@ -212,14 +216,15 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 if (cw == 0 && lg == 0) {  // one wave per pattern block keeps the exponents
                     const int64_t p = (int64_t)tile * 64 + pl;
                     int es = e;
-                    if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
-                    else if (op.kind == KIND_II)
+                    if (KIND == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
+                    else if (KIND == KIND_II)
                         es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
                     expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
                 }
             }
             sp ^= 1;
         }
+        });
         // the next op reads these values from other lanes of this wavefront
         __syncthreads();
     }
@@ -306,11 +311,13 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
 
     for (int k = 0; k < P.nops; k++) {
         const DevOp op = P.ops[k];
+        with_kind(op.kind, [&](auto KC) {
+        constexpr int KIND = decltype(KC)::value;
         const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         int sl = 0, sr = 0;
-        if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
-        if (op.kind == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
+        if (KIND <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
+        if (KIND == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
         floatx4 out[NC][XB];
         float m = 0.0f;
 #pragma unroll
@@ -318,7 +325,7 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
 #pragma unroll
             for (int xb = 0; xb < XB; xb++) out[c][xb] = floatx4{0, 0, 0, 0};
             if (c >= ncat || !active) continue;
-            if (op.kind == KIND_TT) {
+            if (KIND == KIND_TT) {
                 const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
                 const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
 #pragma unroll
@@ -346,14 +353,14 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++)
                         dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(FRm[(xb * KS + ks) * 64 + lane], br, dR[xb], 0, 0, 0);
-                    if (op.kind == KIND_II) {
+                    if (KIND == KIND_II) {
                         const float bl = (y < N) ? L[y * 64] : 0.0f;
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++)
                             dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(FLm[(xb * KS + ks) * 64 + lane], bl, dL[xb], 0, 0, 0);
                     }
                 }
-                if (op.kind == KIND_II) {
+                if (KIND == KIND_II) {
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++) out[c][xb] = dL[xb] * dR[xb];
                 } else {
@@ -390,12 +397,13 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
             }
             if (lg == 0) {
                 int es = e;
-                if (op.kind == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
-                else if (op.kind == KIND_II)
+                if (KIND == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
+                else if (KIND == KIND_II)
                     es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
                 expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
             }
         }
+        });
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }

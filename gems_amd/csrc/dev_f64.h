@@ -106,14 +106,82 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
     }
 }
 
+// Same arithmetic as op_partial for N <= 20, phased so that fewer registers are live:
+// (1) b[x] = eR[x].R, (2) t[x] = (eL[x].L) * b[x] in place, (3) out[j] = Uinv[j].t is
+// stored as soon as it is formed.  Every sum keeps op_partial's order, so the stored
+// values are identical.  Returns max_j |out[j]| (the scaling check is applied to the
+// stored values by the caller).
+template <int N>
+__device__ __forceinline__ double op_partial_store(const KParams &P, const DevOp &op, const double *__restrict__ M,
+                                                   int c, int64_t off, int sl, int sr, double *__restrict__ dst) {
+    static_assert(N <= 20, "phased form is for unrolled state loops");
+    const int ncat = P.ncat, block = N * ncat;
+    const double *__restrict__ part = (const double *)P.part;
+    const cdptr uinv = as_const(opaque(P.inv_evec));
+    const cdptr eL = as_const(M + (int64_t)c * N * N);
+    const cdptr eR = as_const(M + (int64_t)(ncat + c) * N * N);
+    const double *__restrict__ tabs = M + (int64_t)2 * ncat * N * N;
+    double t[N];
+    if (op.kind == KIND_TT) {
+        const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
+        const double *__restrict__ tr = tabs + (int64_t)(P.su + 1) * block + (int64_t)sr * block + c * N;
+#pragma unroll
+        for (int x = 0; x < N; x++) t[x] = tl[x] * tr[x];
+    } else {
+        {
+            const double *__restrict__ R = part + (int64_t)op.right * P.slot_stride + off;
+            double r[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) r[i] = R[i * 64];
+            static_for<0, N>([&](auto X) {
+                constexpr int x = decltype(X)::value;
+                double b = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; i++) b = fma(eR[x * N + i], r[i], b);
+                t[x] = b;
+            });
+        }
+        // keep the left child's loads behind phase (1): the right child's registers are
+        // free by then, which keeps the 20-state kernel at four waves per SIMD
+        __builtin_amdgcn_sched_barrier(0);
+        if (op.kind == KIND_II) {
+            const double *__restrict__ L = part + (int64_t)op.left * P.slot_stride + off;
+            double l[N];
+#pragma unroll
+            for (int i = 0; i < N; i++) l[i] = L[i * 64];
+            static_for<0, N>([&](auto X) {
+                constexpr int x = decltype(X)::value;
+                double a = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; i++) a = fma(eL[x * N + i], l[i], a);
+                t[x] = a * t[x];
+            });
+        } else {
+            const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) t[x] = tl[x] * t[x];
+        }
+    }
+    double m = 0.0;
+    static_for<0, N>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        double o = 0.0;
+#pragma unroll
+        for (int x = 0; x < N; x++) o = fma(uinv[j * N + x], t[x], o);
+        m = fmax(m, fabs(o));
+        dst[j * 64] = o;
+    });
+    return m;
+}
+
 // Fill the branch vectors val0/1/2[c][i] = (1, cof, cof^2) * exp(cof t) * prop_c with
 // cof = lambda_i r_c (tree/phylokernelnew.h:2321-2367, 2731-2766).
-__device__ __forceinline__ void fill_val(const KParams &P, double *v0, double *v1, double *v2) {
+__device__ __forceinline__ void fill_val(const KParams &P, double len, double *v0, double *v1, double *v2) {
     const int n = P.n, block = n * P.ncat;
     for (int idx = threadIdx.x; idx < block; idx += blockDim.x) {
         int c = idx / n, i = idx % n;
         double cof = P.eval[i] * P.rates[c];
-        double val = exp(cof * P.br_len) * P.props[c];
+        double val = exp(cof * len) * P.props[c];
         v0[idx] = val;
         if (v1) {
             double val1 = cof * val;
@@ -140,7 +208,7 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
     __shared__ double s_cat[3][PLK_GPU_MAX_CATS][64];
     __shared__ int s_sc[PLK_GPU_MAX_CATS][64];
 
-    if (MODE != MODE_PARTIALS) fill_val(P, s_val[0], MODE == MODE_THETA ? s_val[1] : nullptr, s_val[2]);
+    if (MODE != MODE_PARTIALS) fill_val(P, P.br_len, s_val[0], MODE == MODE_THETA ? s_val[1] : nullptr, s_val[2]);
 
     // -------- post-order traversal (computePartialLikelihood for every op)
     for (int k = 0; k < P.nops; k++) {
@@ -153,6 +221,24 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
         double wmax = 0.0;
         for (int c = w; c < ncat; c += ncw) {
             const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+            if constexpr (N <= 20) {
+                const double m = op_partial_store<N>(P, op, M, c, off, sl, sr, outp + off);
+                if (P.safe) {
+                    const int64_t a = p * ncat + c;
+                    unsigned sc = 0;
+                    if (op.kind == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + a];
+                    else if (op.kind == KIND_II)
+                        sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + a] + scale[(int64_t)op.right * P.scale_stride + a];
+                    if (op.kind != KIND_TT && m < kST && invar == 0.0) {
+#pragma unroll
+                        for (int j = 0; j < N; j++) outp[off + j * 64] *= kTwo256;
+                        sc += 1;
+                    }
+                    scale[(int64_t)op.out_slot * P.scale_stride + a] = (uint16_t)sc;
+                }
+                wmax = fmax(wmax, m);
+                continue;
+            }
             double out[N];
             op_partial<N>(P, op, M, c, off, sl, sr, out);
             double m = 0.0;
@@ -311,7 +397,7 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
 // (dotProductTriple, tree/phylokernelnew.h:2474-2550) or logL
 // (computeLikelihoodFromBufferSIMD, 3373-3400).
 template <int N, int DERIV>
-__device__ inline void theta_eval_f64_body(const KParams &P, const int tile) {
+__device__ inline bool theta_eval_f64_body(const KParams &P, const int tile, const double len) {
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
     const int ncw = nwaves(), ncat = P.ncat;
@@ -319,7 +405,7 @@ __device__ inline void theta_eval_f64_body(const KParams &P, const int tile) {
     const double *__restrict__ theta = (const double *)P.theta;
     __shared__ double s_val[3][PLK_GPU_MAX_CATS * N];
     __shared__ double s_cat[3][PLK_GPU_MAX_CATS][64];
-    fill_val(P, s_val[0], DERIV ? s_val[1] : nullptr, s_val[2]);
+    fill_val(P, len, s_val[0], DERIV ? s_val[1] : nullptr, s_val[2]);
     __syncthreads();
     for (int c = w; c < ncat; c += ncw) {
         const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
@@ -367,7 +453,86 @@ __device__ inline void theta_eval_f64_body(const KParams &P, const int tile) {
             r0 = wave_sum(p < P.nptn ? l * f : 0.0);
         }
     }
-    finish_tiles(P, r0, r1, tile, P.ntiles);
+    return finish_tiles(P, r0, r1, tile, P.ntiles);
+}
+
+// One step of minimizeNewton (utils/optimization.cpp:422-499) after a derivative
+// evaluation at s.rts returned (d1, d2) = (dlogL/dt, d2logL/dt2): the function is
+// computeFuncDerv's f = -d1, df = -d2 (tree/phylotree.cpp:2555-2562).  Evaluation 0 sets
+// up the bracket; every later one updates it, then the loop body of iteration j+1 picks
+// the next iterate (bisection or Newton step) or finishes.  Sets done when the solve
+// ends; out is the answer when status is NEWTON_OK.  Non-finite derivatives follow the
+// engine's derivative call: NORM reports underflow, SAFE continues with zeros.
+__device__ inline void newton_after_eval(NewtonState &s, double d1, double d2) {
+    s.evals++;
+    if (!isfinite(d1)) {
+        if (!s.safe) {
+            s.status = NEWTON_UNDERFLOW;
+            s.done = 1;
+            return;
+        }
+        d1 = d2 = 0.0;
+    }
+    const double f = -d1, df = -d2;
+    s.f = f;
+    s.df = df;
+    if (!isfinite(f) || !isfinite(df)) {
+        s.status = NEWTON_NONFINITE;
+        s.done = 1;
+        return;
+    }
+    if (s.j == 0) {
+        if (df >= 0.0 && fabs(f) < s.xacc) {
+            s.out = s.rts;
+            s.done = 1;
+            return;
+        }
+        if (f < 0.0) {
+            s.xl = s.rts;
+            s.xh = s.x2;
+        } else {
+            s.xh = s.rts;
+            s.xl = s.x1;
+        }
+        s.dx = fabs(s.xh - s.xl);
+    } else {
+        if (df > 0.0 && fabs(f) < s.xacc) {
+            s.out = s.rts;
+            s.done = 1;
+            return;
+        }
+        if (f < 0.0) s.xl = s.rts;
+        else if (f > 0.0) s.xh = s.rts;
+    }
+    s.j++;
+    if (s.j > s.max_nr) {
+        s.status = NEWTON_MAXITER;
+        s.done = 1;
+        return;
+    }
+    const double rts = s.rts, xl = s.xl, xh = s.xh;
+    s.rts_old = rts;
+    if ((df <= 0.0) || (((rts - xh) * df - f) * ((rts - xl) * df - f) >= 0.0)) {
+        s.dx = 0.5 * (xh - xl);
+        s.rts = xl + s.dx;
+        if (xl == s.rts) {
+            s.out = s.rts;
+            s.done = 1;
+            return;
+        }
+    } else {
+        s.dx = f / df;
+        s.rts = rts - s.dx;
+        if (rts == s.rts) {
+            s.out = s.rts;
+            s.done = 1;
+            return;
+        }
+    }
+    if (fabs(s.dx) < s.xacc || s.j == s.max_nr) {
+        s.out = s.rts_old;
+        s.done = 1;
+    }
 }
 
 }  // namespace dev

@@ -289,6 +289,7 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMallocAsync((void **)&c->d_tile_sums, sizeof(double) * 2 * c->ntiles, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_result, sizeof(double) * 2, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_counter, sizeof(unsigned), c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_newton, sizeof(NewtonState), c->stream));
     const int n = nstates;
     const size_t model_doubles = n + 2 * n * n + 2 * ncat + 2 * (size_t)(c->su + 1) * n + n + n * n;
     CREATE_TRY(hipMallocAsync((void **)&c->d_model, sizeof(double) * model_doubles, c->stream));
@@ -302,6 +303,7 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     c->d_pi = c->d_tipvec + (size_t)(c->su + 1) * n;
     c->d_piU = c->d_pi + n;
     CREATE_TRY(hipHostMalloc((void **)&c->h_result, sizeof(double) * 2, hipHostMallocDefault));
+    CREATE_TRY(hipHostMalloc((void **)&c->h_newton, sizeof(NewtonState), hipHostMallocDefault));
     for (int i = 0; i < 6; i++) CREATE_TRY(hipEventCreate(&c->ev[i]));
     // zero-initialised state: counters, scale counts and partials of unused slots
     CREATE_TRY(hipMemsetAsync(c->d_counter, 0, sizeof(unsigned), c->stream));
@@ -322,12 +324,13 @@ extern "C" int plk_destroy(plk_ctx *c) {
     void *bufs[] = {c->d_part, c->d_scale, c->d_states, c->d_freq, c->d_invar, c->d_mats, c->d_model,
                     c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
                     c->d_result, c->d_counter, c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres,
-                    c->d_mcount};
+                    c->d_mcount, c->d_newton};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, c->stream);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_result) (void)hipHostFree(c->h_result);
+    if (c->h_newton) (void)hipHostFree(c->h_newton);
     for (int i = 0; i < 6; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -495,6 +498,7 @@ extern "C" int plk_branch_loglik(plk_ctx *c, int a, int b, double t, double *log
     bool used[3] = {false, false, false};
     rc = run_traversal(*c, p, MODE_LOGL, used);
     if (rc) return rc;
+    c->ptn_lh_valid = true;
     HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     timing_collect(*c, used);
@@ -558,6 +562,7 @@ extern "C" int plk_loglik_from_theta(plk_ctx *c, double t, double *logl) {
     KParams p = make_params(*c);
     p.br_len = t;
     HIP_TRY(f64_theta_eval(p, 0, c->stream));
+    c->ptn_lh_valid = true;
     HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     double v = c->h_result[0];
@@ -569,6 +574,60 @@ extern "C" int plk_loglik_from_theta(plk_ctx *c, double t, double *logl) {
     }
     *logl = v;
     return PLK_OK;
+}
+
+extern "C" int plk_newton_branch(plk_ctx *c, int a, int b, double x1, double xguess, double x2, double xacc,
+                                 int max_nr, double *out, int *evals) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (!out || !std::isfinite(x1) || !std::isfinite(x2) || !std::isfinite(xguess) || x1 < 0 || x2 < x1 ||
+        !(xacc > 0) || max_nr < 0)
+        return fail(PLK_ERR_ARG, "plk_newton_branch: bad arguments");
+    rc = branch_refs(*c, a, b);
+    if (rc) return rc;
+    NewtonState &h = *c->h_newton;
+    h = NewtonState{};
+    h.x1 = x1;
+    h.x2 = x2;
+    h.xacc = xacc;
+    h.rts = std::min(std::max(xguess, x1), x2);
+    h.max_nr = max_nr;
+    h.safe = c->safe;
+    HIP_TRY(hipMemcpyAsync(c->d_newton, &h, sizeof(NewtonState), hipMemcpyHostToDevice, c->stream));
+    KParams p = make_params(*c);
+    p.br_a = a;
+    p.br_b = b;
+    p.br_len = h.rts;
+    bool used[3] = {false, false, false};
+    // evaluation 0 builds theta (computeFuncDerv's first call never reuses it)
+    rc = run_traversal(*c, p, MODE_THETA, used);
+    if (rc) return rc;
+    c->theta_valid = true;
+    c->theta_a = a;
+    c->theta_b = b;
+    HIP_TRY(newton_first(c->d_newton, c->d_result, c->stream));
+    // queue iterations in growing rounds; a finished solve turns the rest into no-ops
+    // (iteration j evaluates at most once and iteration max_nr never does, so max_nr
+    // launches always suffice)
+    int round = std::min(6, std::max(max_nr, 1)), launched = 0;
+    for (;;) {
+        for (int k = 0; k < round; k++) HIP_TRY(f64_theta_newton(p, c->d_newton, c->stream));
+        launched += round;
+        HIP_TRY(hipMemcpyAsync(&h, c->d_newton, sizeof(NewtonState), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (h.done) break;
+        if (launched > max_nr) return fail(PLK_ERR_STATE, "plk_newton_branch: solve did not finish");
+        round = std::min(2 * round, 48);
+    }
+    timing_collect(*c, used);
+    if (evals) *evals = h.evals;
+    switch (h.status) {
+        case NEWTON_OK: *out = h.out; return PLK_OK;
+        case NEWTON_UNDERFLOW: return fail(PLK_ERR_NUMERIC, "Numerical underflow (lh-derivative): use SAFE scaling");
+        case NEWTON_NONFINITE: return fail(PLK_ERR_NUMERIC, "Wrong computeFuncDerv");
+        default: return fail(PLK_ERR_NUMERIC, "Maximum number of iterations exceeded in minimizeNewton");
+    }
 }
 
 extern "C" int plk_get_partial(plk_ctx *c, int slot, double *partial, uint16_t *scale) {

@@ -65,6 +65,17 @@ struct KParams {
     double *result;           // 2
 };
 
+// Device-resident state of one Newton branch-length solve (Optimization::minimizeNewton,
+// utils/optimization.cpp:422-499, with computeFuncDerv tree/phylotree.cpp:2555-2562 as the
+// function).  The step logic runs on the device after every derivative evaluation, so a
+// whole solve is a queue of launches with no host round trip in between.
+struct NewtonState {
+    double x1, x2, xacc;                            // bounds and tolerance
+    double rts, rts_old, xl, xh, dx, f, df, out;    // iterate, bracket, last step, -d1, -d2
+    int32_t max_nr, j, done, status, evals, safe;
+};
+enum : int { NEWTON_OK = 0, NEWTON_UNDERFLOW = 1, NEWTON_NONFINITE = 2, NEWTON_MAXITER = 3 };
+
 // Host-side context (opaque plk_ctx).
 struct Ctx {
     int dev = 0;
@@ -88,6 +99,7 @@ struct Ctx {
     void *d_theta = nullptr;
     double *d_buf_scale = nullptr, *d_ptn_lh = nullptr, *d_tile_sums = nullptr, *d_result = nullptr;
     unsigned *d_counter = nullptr;
+    NewtonState *d_newton = nullptr, *h_newton = nullptr;  // device state, pinned host mirror
     // host staging (pinned)
     void *h_stage = nullptr;
     size_t h_stage_cap = 0;
@@ -96,6 +108,7 @@ struct Ctx {
     std::vector<PJob> jobs;
     int theta_a = 0, theta_b = 0;
     bool theta_valid = false;
+    bool ptn_lh_valid = false;  // d_ptn_lh holds the per-pattern logL of the last logL evaluation
     bool timing = false;
     int wpc = 1;  // FP32 matrix-core tile shape (plk_set_tile_shape)
     // many-branch derivative buffers (plk_branch_derivs_multi)
@@ -120,6 +133,8 @@ namespace plk {
 hipError_t f64_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
 hipError_t f64_traverse(const KParams &p, int mode, hipStream_t s);
 hipError_t f64_theta_eval(const KParams &p, int deriv, hipStream_t s);
+hipError_t newton_first(NewtonState *st, const double *result, hipStream_t s);
+hipError_t f64_theta_newton(const KParams &p, NewtonState *st, hipStream_t s);
 hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
 hipError_t f32_traverse(const KParams &p, int mode, hipStream_t s);
 hipError_t f32_theta(const KParams &p, hipStream_t s);

@@ -9,14 +9,37 @@ __global__ void __launch_bounds__(256) k_pmatrix_f64(KParams P, const PJob *__re
     pmatrix_f64_body(P, jobs[blockIdx.x]);
 }
 
+// Occupancy floor: the 20-state partial/logL kernels are held to 128 VGPRs (4 waves per
+// SIMD, a few bytes of spill), so that e.g. 782 tiles x 4 waves of a 50k-pattern
+// alignment fit the chip in one round instead of spilling 14 workgroups into a second.
+// The theta variant keeps its full register budget (it would spill far more).
 template <int N, int MODE>
-__global__ void __launch_bounds__(256) k_traverse_f64(KParams P) {
+constexpr int traverse_min_waves() { return (N == 20 && MODE != MODE_THETA) ? 4 : 1; }
+
+template <int N, int MODE>
+__global__ void __launch_bounds__(256, (traverse_min_waves<N, MODE>())) k_traverse_f64(KParams P) {
     traverse_f64_body<N, MODE>(P, blockIdx.x);
 }
 
 template <int N, int DERIV>
 __global__ void __launch_bounds__(256) k_theta_eval_f64(KParams P) {
-    theta_eval_f64_body<N, DERIV>(P, blockIdx.x);
+    theta_eval_f64_body<N, DERIV>(P, blockIdx.x, P.br_len);
+}
+
+// First Newton step, after the traversal that built theta left (d1, d2) in P.result.
+__global__ void k_newton_first(NewtonState *st, const double *result) {
+    if (threadIdx.x == 0) newton_after_eval(*st, result[0], result[1]);
+}
+
+// One Newton iteration: derivatives at the state's current iterate, then (in the
+// workgroup that finishes the reduction) the step.  A finished solve makes every
+// later launch of the queue return at once.
+template <int N>
+__global__ void __launch_bounds__(256) k_theta_newton_f64(KParams P, NewtonState *st) {
+    if (st->done) return;
+    const double len = st->rts;
+    const bool last = theta_eval_f64_body<N, 1>(P, blockIdx.x, len);
+    if (last && threadIdx.x == 0) newton_after_eval(*st, P.result[0], P.result[1]);
 }
 
 template <int N>
@@ -36,7 +59,27 @@ hipError_t theta_eval_n(const KParams &p, int deriv, hipStream_t s) {
     return hipGetLastError();
 }
 
+template <int N>
+hipError_t theta_newton_n(const KParams &p, NewtonState *st, hipStream_t s) {
+    hipLaunchKernelGGL((k_theta_newton_f64<N>), dim3(p.ntiles), dim3(64 * p.ncw), 0, s, p, st);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+hipError_t newton_first(NewtonState *st, const double *result, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_first, dim3(1), dim3(64), 0, s, st, result);
+    return hipGetLastError();
+}
+
+hipError_t f64_theta_newton(const KParams &p, NewtonState *st, hipStream_t s) {
+    switch (p.n) {
+        case 4: return theta_newton_n<4>(p, st, s);
+        case 20: return theta_newton_n<20>(p, st, s);
+        case 61: return theta_newton_n<61>(p, st, s);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t f64_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s) {
     hipLaunchKernelGGL(k_pmatrix_f64, dim3(njobs), dim3(256), 0, s, p, jobs);

@@ -25,6 +25,9 @@ class PhyloTreeGPU {
     std::vector<char> computed;  // per node (LM_PER_NODE) or per slot (all_dirs)
     std::vector<plk_op> ops;
     int64_t n_derivs = 0, n_ops = 0;
+    // Newton branch solves run device-resident (plk_newton_branch) unless switched to
+    // the host loop below, which issues one derivative call per iteration.
+    bool device_newton = true;
 
     bool is_leaf(int v) const { return v < ntaxa; }
 
@@ -252,7 +255,17 @@ class PhyloTreeGPU {
     int optimize_one(int n1, int n2, double min_bl, double max_bl, int max_nr, double &optx) {
         if (dir(n1, n2) < 0) return fail(PLK_ERR_ARG, "no such branch");
         double current = length(n1, n2);
-        int rc = minimize_newton(n1, n2, min_bl, current, max_bl, min_bl, max_nr, optx);
+        int rc;
+        if (device_newton) {
+            rc = plan(n1, n2);
+            if (rc) return rc;
+            int evals = 0;
+            rc = plk_newton_branch(ctx, ref_dir(n2, n1), ref_dir(n1, n2), min_bl, current, max_bl, min_bl, max_nr,
+                                   &optx, &evals);
+            n_derivs += evals;
+        } else {
+            rc = minimize_newton(n1, n2, min_bl, current, max_bl, min_bl, max_nr, optx);
+        }
         if (rc) return rc;
         int da = dir(n1, n2), db = dir(n2, n1);
         if (optx > max_bl * 0.95) {
@@ -480,6 +493,12 @@ extern "C" int plk_tree_get_partial(plk_tree *t, int dad, int node, double *part
         if (rc) return rc;
     }
     return plk_get_partial(t->ctx, t->slot_of(dad, node), partial, scale);
+}
+
+extern "C" int plk_tree_set_newton(plk_tree *t, int device_resident) {
+    if (!t) return fail(PLK_ERR_ARG, "plk_tree_set_newton: null tree");
+    t->device_newton = device_resident != 0;
+    return PLK_OK;
 }
 
 extern "C" int plk_tree_optimize_one_branch(plk_tree *t, int n1, int n2, double min_bl, double max_bl, int max_nr,

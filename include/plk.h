@@ -124,6 +124,17 @@ int plk_branch_derivs_multi(plk_ctx *ctx, int nbr, const int *a, const int *b, c
  * theta buffer of the last plk_branch_derivs call. */
 int plk_loglik_from_theta(plk_ctx *ctx, double t, double *logl);
 
+/* Optimization::minimizeNewton (utils/optimization.cpp:422-499) on the length of branch
+ * (a, b), with computeFuncDerv (tree/phylotree.cpp:2555-2562, f = -dlogL/dt) as the function,
+ * run device-resident: evaluation 0 is a full traversal building theta at
+ * clamp(xguess, x1, x2); every later evaluation reuses theta, and the bracket/step logic
+ * runs on the device after each one, so iterations are queued launches with no host
+ * round trip.  The arithmetic, stopping rules and error messages are those of the
+ * reference routine; *out receives the minimiser, *evals (nullable) the number of
+ * derivative evaluations.  theta stays valid for plk_loglik_from_theta afterwards. */
+int plk_newton_branch(plk_ctx *ctx, int a, int b, double x1, double xguess, double x2, double xacc,
+                      int max_nr, double *out, int *evals);
+
 /* Read back one slot: partial in canonical order [ptn][c][x] as FP64 (the FP32 form
  * returns its own representation), and its scale counts (NORM: nptn uint16; SAFE:
  * nptn*ncat uint16 [ptn][c]).  Either pointer may be NULL. */

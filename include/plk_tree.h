@@ -52,6 +52,10 @@ int plk_tree_loglik_from_theta(plk_tree *t, int dad, int node, double *logl);
 int plk_tree_get_partial(plk_tree *t, int dad, int node, double *partial, uint16_t *scale);
 int plk_tree_optimize_one_branch(plk_tree *t, int node1, int node2, double min_bl, double max_bl,
                                  int max_nr_step, double *optx);
+/* Newton solves of optimize_one/all_branches: device_resident != 0 (default) uses
+ * plk_newton_branch; 0 runs minimizeNewton on the host with one plk_branch_derivs call per
+ * iteration.  Both give the same lengths. */
+int plk_tree_set_newton(plk_tree *t, int device_resident);
 int plk_tree_optimize_all_branches(plk_tree *t, int iterations, double tol, double min_bl, double max_bl,
                                    int max_nr_step, double *logl);
 /* Invalidate every partial and run one full traversal + branch logL (the benchmark step). */

@@ -198,3 +198,56 @@ def test_all_branch_derivs(prec, safe):
     for (a, b), d1 in zip(br2, df2):
         assert d1 == single.derivs(a, b)[0]
     assert g.loglik(*br[0]) == single.loglik(*br[0])
+
+
+NEWTON_CASES = [
+    (G.FP64, 0, G.SEQ_DNA, 2000),
+    (G.FP64, 1, G.SEQ_PROTEIN, 1500),
+    (G.FP32, 0, G.SEQ_PROTEIN, 1500),
+    (G.FP64, 1, G.SEQ_CODON, 300),
+]
+
+
+@pytest.mark.parametrize("case", NEWTON_CASES, ids=["dna-f64", "protein-f64-safe", "protein-f32", "codon-f64-safe"])
+def test_device_newton_matches_host_loop(case):
+    """The device-resident Newton solve takes the same steps as minimizeNewton driven from
+    the host one derivative call at a time: identical lengths, evaluation counts and logL."""
+    prec, safe, seq, sites = case
+    p = G.Problem(seq, 14, sites, seed=21)
+    say("device newton", prec, safe, seq)
+    dev = G.Tree(p, precision=prec, scaling=safe)
+    host = G.Tree(p, precision=prec, scaling=safe)
+    host.set_newton(False)
+    o = S.Oracle(p, safe=bool(safe)) if prec == G.FP64 else None
+    br = branches(p)
+    for (u, v) in br[:6]:
+        a = dev.optimize_one(u, v)
+        b = host.optimize_one(u, v)
+        assert a == b, (u, v, a, b)
+        if o is not None:  # the oracle's minimizeNewton on the same sequence of solves
+            # both stop once a step is below xacc (= min_bl, 1e-6) from an iterate that
+            # brackets the same root, so they agree to 2 * xacc
+            t_orc = o.optimize_one(u, v)
+            assert abs(a - t_orc) <= 2e-6, (u, v, a, t_orc)
+    assert dev.counters()[1] == host.counters()[1]  # derivative evaluations
+    assert dev.loglik(*br[0]) == host.loglik(*br[0])
+
+
+def test_device_newton_abi_errors():
+    import ctypes
+    p = G.Problem(G.SEQ_DNA, 8, 400, seed=3)
+    g = G.Tree(p)
+    g.full_traversal()
+    ctx = g.context()
+    u, v = branches(p)[0]
+    # node references of the branch as the tree layer passes them
+    ra, rb = -(u + 1) if u < p.ntaxa else u - p.ntaxa, -(v + 1) if v < p.ntaxa else v - p.ntaxa
+    out, ev = ctypes.c_double(), ctypes.c_int()
+    L = G.lib()
+    assert L.plk_newton_branch(ctx, ra, rb, 1e-6, 0.1, 1e-7, 1e-6, 100, ctypes.byref(out), None) == -1  # x2 < x1
+    assert L.plk_newton_branch(ctx, ra, rb, 1e-6, 0.1, 10.0, 0.0, 100, ctypes.byref(out), None) == -1  # xacc
+    # max_nr = 0: the reference loop never runs and reports the iteration limit
+    rc = L.plk_newton_branch(ctx, ra, rb, 1e-6, 0.1, 10.0, 1e-6, 0, ctypes.byref(out), ctypes.byref(ev))
+    assert rc == -4 and b"Maximum number of iterations" in L.plk_last_error()
+    rc = L.plk_newton_branch(ctx, ra, rb, 1e-6, 0.1, 10.0, 1e-6, 100, ctypes.byref(out), ctypes.byref(ev))
+    assert rc == 0 and 1e-6 <= out.value <= 10.0 and ev.value >= 1
