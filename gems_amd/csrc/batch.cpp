@@ -257,6 +257,7 @@ extern "C" int plk_batch_loglik(plk_batch *b, const int *a, const int *bref, con
             }
         }
         if (logl) logl[i] = v;
+        c->ptn_lh_valid = true;
         sum += v;
         c->updates += (int64_t)c->pending.size() * c->nptn;
         b->updates += (int64_t)c->pending.size() * c->nptn;

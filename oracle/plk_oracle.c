@@ -887,3 +887,37 @@ void orc_tip_lh(int n, int seq_type, const double *inv_evec, double *out) {
         }
     }
 }
+
+/* RELL loop of IQTree::saveCurrentTree (tree/iqtree.cpp:3852-3931): pattern_lh is the
+ * per-pattern logL rounded to float (3867-3870; a non-finite value first gets the branch
+ * logL fallback, tree/phylokernelnew.h:3167-3184), rell[s] = sum_p pattern_lh[p] *
+ * boot_samples[s][p] (dotProduct, 3912-3915), summed here sequentially in FP64.  With
+ * uniform != NULL the replicate state is updated by the rule of 3917-3931, the caller's
+ * uniform[s] standing in for random_double(rstream). */
+void orc_rell_save_tree(int nsamples, int nptn, const float *counts, const double *ptn_lh, double cur_logl,
+                        double eps, const double *uniform, double *rell, double *boot_logl, int32_t *boot_counts,
+                        double *boot_orig, int32_t *improved) {
+    float *lh = (float *)malloc(sizeof(float) * (size_t)(nptn > 0 ? nptn : 1));
+    for (int p = 0; p < nptn; p++) {
+        double l = ptn_lh[p];
+        if (!isfinite(l)) l = LOG_SCALING_THRESHOLD * 4;
+        lh[p] = (float)l;
+    }
+    for (int s = 0; s < nsamples; s++) {
+        const float *w = counts + (size_t)s * nptn;
+        double r = 0.0;
+        for (int p = 0; p < nptn; p++) r += (double)lh[p] * (double)w[p];
+        if (rell) rell[s] = r;
+        if (!uniform) continue;
+        int better = r > boot_logl[s] + eps;
+        if (!better && r > boot_logl[s] - eps) better = uniform[s] <= 1.0 / (boot_counts[s] + 1);
+        if (better) {
+            if (r <= boot_logl[s] + eps) boot_counts[s]++;
+            else boot_counts[s] = 1;
+            boot_logl[s] = boot_logl[s] > r ? boot_logl[s] : r;
+            boot_orig[s] = cur_logl;
+        }
+        if (improved) improved[s] = better;
+    }
+    free(lh);
+}

@@ -21,6 +21,7 @@
  *   optimizeAllBranches           tree/phylotree.cpp:2699-2801, tree/mtree.cpp:3040-3082
  *   tip vectors                   tree/phylotreesse.cpp:354-375, model/modelsubst.cpp:183-208,
  *                                 model/modeldna.cpp:462-476, model/modelprotein.cpp:1240-1256
+ *   RELL / UFBoot replicate update tree/iqtree.cpp:3852-3931
  *
  * Memory layout follows the reference's pattern-interleaved SIMD layout with
  * ORC_V lanes: element (ptn, c, x) of a partial sits at double offset
@@ -34,6 +35,8 @@
  *   from-theta logL  equals branch logL at the same length
  *   Newton           converges to the finite-difference optimum (df ~ 0, ddf < 0)
  *   tip vectors      closed form Uinv * indicator(state set)
+ *   RELL             counts = pattern frequencies reproduce the tree logL (to the float
+ *                    rounding of pattern_lh); the update rule on a hand-worked sequence
  */
 #ifndef PLK_ORACLE_H
 #define PLK_ORACLE_H
@@ -84,6 +87,11 @@ double orc_optimize_all_branches(orc_tree *t, int iterations, double tol, double
                                  double max_bl, int max_nr_step);
 
 void orc_tip_lh(int nstates, int seq_type, const double *inv_evec, double *out);
+/* UFBoot RELL scoring and replicate update (IQTree::saveCurrentTree); uniform == NULL
+ * scores only.  counts: nsamples x nptn. */
+void orc_rell_save_tree(int nsamples, int nptn, const float *counts, const double *ptn_lh, double cur_logl,
+                        double eps, const double *uniform, double *rell, double *boot_logl, int32_t *boot_counts,
+                        double *boot_orig, int32_t *improved);
 int orc_state_unknown(int nstates, int seq_type);
 
 #ifdef __cplusplus

@@ -69,6 +69,9 @@ def oracle_lib(lanes=None):
                                                         ctypes.c_double, ctypes.c_int]),
         "orc_tip_lh": (None, [ctypes.c_int, ctypes.c_int, dp, dp]),
         "orc_state_unknown": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+        "orc_rell_save_tree": (None, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), dp, ctypes.c_double,
+                                      ctypes.c_double, dp, dp, dp, ctypes.POINTER(ctypes.c_int32), dp,
+                                      ctypes.POINTER(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -437,3 +440,26 @@ def brute_force_loglik(prob, a, b):
         total += prob.props[c] * lc
     lp = np.log(total)
     return lp, float(np.dot(lp, prob.freq))
+
+
+class OracleRell:
+    """UFBoot replicate state scored by the oracle's RELL loop (orc_rell_save_tree)."""
+
+    def __init__(self, counts):
+        self.counts = np.ascontiguousarray(counts, dtype=np.float32)
+        self.ns, self.nptn = self.counts.shape
+        self.logl = np.full(self.ns, -np.finfo(np.float64).max)
+        self.orig = np.full(self.ns, -np.finfo(np.float64).max)
+        self.best_counts = np.zeros(self.ns, dtype=np.int32)
+
+    def save_tree(self, ptn_lh, cur_logl, eps=0.5, uniform=None):
+        lib = oracle_lib()
+        ptn_lh = np.ascontiguousarray(ptn_lh, dtype=np.float64)
+        rell = np.zeros(self.ns)
+        improved = np.zeros(self.ns, dtype=np.int32)
+        i32 = ctypes.POINTER(ctypes.c_int32)
+        u = None if uniform is None else _dp(np.ascontiguousarray(uniform, dtype=np.float64))
+        lib.orc_rell_save_tree(self.ns, self.nptn, self.counts.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                               _dp(ptn_lh), cur_logl, eps, u, _dp(rell), _dp(self.logl),
+                               self.best_counts.ctypes.data_as(i32), _dp(self.orig), improved.ctypes.data_as(i32))
+        return rell, improved

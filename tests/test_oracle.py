@@ -179,3 +179,39 @@ def test_planted_mistake_is_caught():
     bad.Q = p.Q.T.copy()  # wrong orientation of the rate matrix
     _, ref_bad = S.pruning_loglik(bad, a, b)
     assert abs(good - ref_bad) > 1e-6 * abs(good)
+
+
+def test_rell_pattern_frequencies_reproduce_logl():
+    """A replicate whose counts are the pattern frequencies is the original alignment:
+    its RELL score is the tree logL, up to the float rounding of each pattern logL."""
+    rng = np.random.default_rng(31)
+    p = S.make_problem(4, 8, 400, rng)
+    o = S.Oracle(p)
+    a, b = 0, int(p.nei[0, 0])
+    tot, lp = o.loglik(a, b, with_patterns=True)
+    freq = np.asarray(p.freq, dtype=np.float64)
+    counts = np.stack([freq, 2 * freq, np.zeros_like(freq)]).astype(np.float32)
+    r = S.OracleRell(counts)
+    rell, _ = r.save_tree(lp, tot)
+    bound = float(np.sum(np.abs(freq * lp))) * 2.0 ** -24
+    assert abs(rell[0] - tot) <= bound
+    assert abs(rell[1] - 2 * tot) <= 2 * bound
+    assert rell[2] == 0.0
+
+
+def test_rell_update_rule_hand_worked():
+    """saveCurrentTree's replicate rule (tree/iqtree.cpp:3917-3931) on a sequence of
+    scores worked out by hand: one replicate, one pattern of count 1, eps = 0.5."""
+    r = S.OracleRell(np.ones((1, 1), dtype=np.float32))
+    seq = [
+        # (pattern logL, uniform, expected improved, best logL, count, orig)
+        (-100.0, 0.99, 1, -100.0, 1, -100.0),   # first tree always wins (best = -DBL_MAX)
+        (-99.75, 0.40, 1, -99.75, 2, -99.75),   # tie within eps: 0.40 <= 1/2 -> count 2
+        (-99.5, 0.40, 0, -99.75, 2, -99.75),    # tie: 0.40 > 1/3 -> kept
+        (-99.5, 0.25, 1, -99.5, 3, -99.5),      # tie: 0.25 <= 1/3 -> count 3, max taken
+        (-98.0, 0.99, 1, -98.0, 1, -98.0),      # clearly better -> count reset to 1
+        (-98.6, 0.00, 0, -98.0, 1, -98.0),      # clearly worse (below best - eps)
+    ]
+    for lh, u, imp, best, cnt, orig in seq:
+        _, improved = r.save_tree(np.array([lh]), lh, eps=0.5, uniform=np.array([u]))
+        assert improved[0] == imp and r.logl[0] == best and r.best_counts[0] == cnt and r.orig[0] == orig, lh
