@@ -28,6 +28,23 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+PROFILE = os.path.join(ROOT, "profiles", "k_traverse_f32_mfma_protein20_100taxa_50k.md")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the benchmark kernel, from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE counter passes, each scaled by what the counter reads for a
+    1 GiB device copy in the same run)."""
+    try:
+        with open(PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_baseline(p, seconds=10.0, min_reps=2):
     """Time the oracle's full traversal + branch logL on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -109,6 +126,7 @@ def main():
         return {"ups": ups, "ms_per_step": el / args.steps * 1e3, "kernel_avg_ms": k_avg_s * 1e3,
                 "achieved": achieved, "bytes_per_update": bpu, "logl": logl, "tile_waves_per_category": tile}
 
+    traffic = profiled_traffic() if (args.taxa, args.sites) == (100, 50000) else None
     copy_gbps = G.probe_copy(1 << 30, 5)
     log(f"measured device copy bandwidth: {copy_gbps:.0f} GB/s")
 
@@ -152,7 +170,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": f32["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": f32["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "frac": f32["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": os.path.relpath(PROFILE, ROOT) if traffic else None,
+            "traffic_over_algorithmic": traffic / (upd_per_step * f32["bytes_per_update"]) if traffic else None,
             "kernel": "k_traverse_f32_mfma<20,MODE_LOGL>", "kernel_avg_ms": f32["kernel_avg_ms"],
             "alg_bytes_per_update": f32["bytes_per_update"],
             "measured_copy_GBps": copy_gbps, "frac_of_measured_copy": f32["achieved"] / copy_gbps,

@@ -75,6 +75,14 @@ _SIGS = {
     "plk_branch_derivs": (_int, [_P, _int, _int, _dbl, _int, _dp, _dp]),
     "plk_loglik_from_theta": (_int, [_P, _dbl, _dp]),
     "plk_newton_branch": (_int, [_P, _int, _int, _dbl, _dbl, _dbl, _dbl, _int, _dp, _ip]),
+    "plk_newton_multi": (_int, [_P, _int, _ip, _ip, _dbl, _dp, _dbl, _dbl, _int, _dp, _dp, _ip]),
+    # alignment ingest
+    "plk_align_read": (_int, [ctypes.POINTER(_P), ctypes.c_char_p, _int, _int]),
+    "plk_align_parse": (_int, [ctypes.POINTER(_P), ctypes.c_char_p, ctypes.c_size_t, _int, _int]),
+    "plk_align_destroy": (_int, [_P]),
+    "plk_align_dims": (_int, [_P, _ip]),
+    "plk_align_name": (_int, [_P, _int, ctypes.c_char_p, _int]),
+    "plk_align_patterns": (_int, [_P, _u8p, _dp, _ip, _ip]),
     # UFBoot RELL
     "plk_rell_create": (_int, [ctypes.POINTER(_P), _int, _int, ctypes.POINTER(ctypes.c_float)]),
     "plk_rell_destroy": (_int, [_P]),
@@ -88,6 +96,7 @@ _SIGS = {
     "plk_synchronize": (_int, [_P]),
     "plk_get_exponents": (_int, [_P, _int, _ip]),
     "plk_set_tile_shape": (_int, [_P, _int]),
+    "plk_set_level_waves": (_int, [_P, _int]),
     "plk_probe_copy": (_int, [ctypes.c_int64, _int, _dp]),
     # model / synthetic data
     "plk_state_unknown": (_int, [_int, _int]),
@@ -117,6 +126,7 @@ _SIGS = {
     "plk_tree_get_partial": (_int, [_P, _int, _int, _dp, _u16p]),
     "plk_tree_optimize_one_branch": (_int, [_P, _int, _int, _dbl, _dbl, _int, _dp]),
     "plk_tree_set_newton": (_int, [_P, _int]),
+    "plk_tree_nni_evaluate": (_int, [_P, _int, _ip, _ip, _dbl, _dbl, _int, _dp, _dp]),
     "plk_tree_optimize_all_branches": (_int, [_P, _int, _dbl, _dbl, _dbl, _int, _dp]),
     "plk_tree_full_traversal": (_int, [_P, _dp]),
     "plk_tree_counters": (_int, [_P, _i64p]),
@@ -305,6 +315,27 @@ class Tree:
         _check(lib().plk_tree_optimize_one_branch(self.h, a, b, min_bl, max_bl, max_nr, ctypes.byref(v)))
         return v.value
 
+    def nni_evaluate(self, branches, min_bl=1e-6, max_bl=10.0, max_nr=10):
+        """Both NNI5 moves of each internal branch (u, v): (scores[nbr, 2], lens[nbr, 2, 5])."""
+        n1 = np.array([b[0] for b in branches], dtype=np.int32)
+        n2 = np.array([b[1] for b in branches], dtype=np.int32)
+        sc = np.zeros(2 * len(branches))
+        ln = np.zeros(10 * len(branches))
+        _check(lib().plk_tree_nni_evaluate(self.h, len(branches), n1.ctypes.data_as(_ip), n2.ctypes.data_as(_ip),
+                                           min_bl, max_bl, max_nr, _d(sc), _d(ln)))
+        return sc.reshape(-1, 2), ln.reshape(-1, 2, 5)
+
+    def newton_multi(self, refs, guesses, min_bl=1e-6, max_bl=10.0, max_nr=100):
+        """plk_newton_multi on node-reference pairs: (lengths, logls, status)."""
+        a = np.array([r[0] for r in refs], dtype=np.int32)
+        b = np.array([r[1] for r in refs], dtype=np.int32)
+        g = np.ascontiguousarray(guesses, dtype=np.float64)
+        out, lg = np.zeros(len(refs)), np.zeros(len(refs))
+        st = np.zeros(len(refs), dtype=np.int32)
+        _check(lib().plk_newton_multi(self.context(), len(refs), a.ctypes.data_as(_ip), b.ctypes.data_as(_ip), min_bl,
+                                      _d(g), max_bl, min_bl, max_nr, _d(out), _d(lg), st.ctypes.data_as(_ip)))
+        return out, lg, st
+
     def set_newton(self, device_resident):
         """Newton solves on the device (True, default) or as a host loop (False)."""
         _check(lib().plk_tree_set_newton(self.h, 1 if device_resident else 0))
@@ -333,6 +364,9 @@ class Tree:
 
     def set_tile_shape(self, wpc):
         _check(lib().plk_set_tile_shape(self.context(), wpc))
+
+    def set_level_waves(self, waves):
+        _check(lib().plk_set_level_waves(self.context(), waves))
 
     def set_timing(self, on):
         _check(lib().plk_set_timing(self.context(), int(on)))
@@ -510,3 +544,34 @@ class Rell:
             self.close()
         except Exception:
             pass
+
+
+class Alignment:
+    """PHYLIP / FASTA alignment compressed to site patterns (plk_align_*).  fmt: 0 detect,
+    1 PHYLIP, 2 PHYLIP sequential, 3 FASTA; seq_type None detects DNA / protein."""
+
+    def __init__(self, path=None, text=None, fmt=0, seq_type=None):
+        h = _P()
+        st = -1 if seq_type is None else seq_type
+        if text is not None:
+            raw = text.encode() if isinstance(text, str) else bytes(text)
+            _check(lib().plk_align_parse(ctypes.byref(h), raw, len(raw), fmt, st))
+        else:
+            _check(lib().plk_align_read(ctypes.byref(h), str(path).encode(), fmt, st))
+        try:
+            d = np.zeros(6, dtype=np.int32)
+            _check(lib().plk_align_dims(h, d.ctypes.data_as(_ip)))
+            self.ntaxa, self.nsites, self.nptn, self.n, self.seq_type, self.gaps_only = [int(x) for x in d]
+            buf = ctypes.create_string_buffer(4096)
+            self.names = []
+            for i in range(self.ntaxa):
+                _check(lib().plk_align_name(h, i, buf, len(buf)))
+                self.names.append(buf.value.decode())
+            self.states = np.zeros((self.ntaxa, self.nptn), dtype=np.uint8)
+            self.freq = np.zeros(self.nptn)
+            self.site_pattern = np.zeros(self.nsites, dtype=np.int32)
+            self.flags = np.zeros(self.nptn, dtype=np.int32)
+            _check(lib().plk_align_patterns(h, self.states.ctypes.data_as(_u8p), _d(self.freq),
+                                            self.site_pattern.ctypes.data_as(_ip), self.flags.ctypes.data_as(_ip)))
+        finally:
+            lib().plk_align_destroy(h)

@@ -77,6 +77,30 @@ def all_branch_derivs(p, precision, scaling=G.NORM, reps=20):
     return br, np.stack([df, ddf], axis=1), el
 
 
+def nni_pass(p, precision, reps=3):
+    """Both NNI5 moves of every internal branch (getBestNNIForBran on all of them), batched."""
+    g = G.Tree(p, precision=precision, all_directions=2)
+    T = p.ntaxa
+    ib = [(u, int(v)) for u in range(T, 2 * T - 2) for v in p.nei[u] if v > u]
+    g.nni_evaluate(ib)  # warm-up (all directional partials)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sc, _ = g.nni_evaluate(ib)
+    el = (time.perf_counter() - t0) / reps
+    g.close()
+    return ib, sc, el
+
+
+def nni_cpu(p, ib, threads):
+    import support as S
+    o = S.Oracle(p, nthreads=threads)
+    t0 = time.perf_counter()
+    sc = np.array([[o.nni_eval(u, v, m)[0] for m in (0, 1)] for (u, v) in ib])
+    el = time.perf_counter() - t0
+    o.close()
+    return sc, el
+
+
 def config1(quick):
     p = G.Problem(G.SEQ_DNA, 32, 10000, mean_len=0.1, alpha=0.5, seed=1)
     log(f"config1: {p.nptn} patterns")
@@ -91,8 +115,17 @@ def config1(quick):
     do = np.array([o.derivs(u, v)[:2] for (u, v) in br])
     e64 = float(np.max(np.abs(d64 - do) / np.maximum(1.0, np.abs(do))))
     e32 = float(np.max(np.abs(d32 - do) / np.maximum(1.0, np.abs(do))))
+    ib, n64, tn64 = nni_pass(p, G.FP64)
+    _, n32, tn32 = nni_pass(p, G.FP32)
+    nco, tnc = nni_cpu(p, ib, thr)
+    nni = {"branches": len(ib), "moves": 2 * len(ib), "fp64_ms": tn64 * 1e3, "fp32_ms": tn32 * 1e3,
+           "cpu_oracle_ms": tnc * 1e3, "cpu_threads": thr,
+           "score_max_rel_err_fp64": float(np.max(np.abs(n64 - nco) / np.abs(nco))),
+           "score_max_rel_err_fp32": float(np.max(np.abs(n32 - nco) / np.abs(nco))),
+           "best_move_agrees_fp64": bool(np.array_equal(np.argmax(n64, axis=1), np.argmax(nco, axis=1)))}
     return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
             "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
+            "nni5_all_branches": nni,
             "all_branch_derivs_ms": {"fp64": t64 * 1e3, "fp32": t32 * 1e3, "branches": len(br),
                                      "updates_per_pass": 3 * (p.ntaxa - 2) * p.nptn,
                                      "fp64_updates_per_s": 3 * (p.ntaxa - 2) * p.nptn / t64,

@@ -53,6 +53,49 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
+// Renormalisation exponent of a pattern maximum: m * 2^-e in [0.5, 1).
+__device__ __forceinline__ int norm_exponent(float m) {
+    int e = 0;
+    if (m > 0.0f) (void)frexpf(m, &e);
+    return e;
+}
+
+// Three-value form of finish_tiles (tile_sums stride 3, result[0..2]).
+__device__ inline bool finish_tiles3(const KParams &P, double v0, double v1, double v2, int tile, int ntiles) {
+    __shared__ int s_last3;
+    if (threadIdx.x == 0) {
+        P.tile_sums[3 * tile] = v0;
+        P.tile_sums[3 * tile + 1] = v1;
+        P.tile_sums[3 * tile + 2] = v2;
+        __threadfence();
+        unsigned prev = atomicAdd(P.counter, 1u);
+        s_last3 = (prev == (unsigned)ntiles - 1);
+    }
+    __syncthreads();
+    if (s_last3) {
+        __threadfence();
+        if (threadIdx.x < 64) {
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int i = threadIdx.x; i < ntiles; i += 64) {
+                a0 += __hip_atomic_load(&P.tile_sums[3 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                a1 += __hip_atomic_load(&P.tile_sums[3 * i + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                a2 += __hip_atomic_load(&P.tile_sums[3 * i + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            a0 = wave_sum(a0);
+            a1 = wave_sum(a1);
+            a2 = wave_sum(a2);
+            if (threadIdx.x == 0) {
+                P.result[0] = a0;
+                P.result[1] = a1;
+                P.result[2] = a2;
+                __hip_atomic_store(P.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    __syncthreads();
+    return s_last3;
+}
+
 // f(integral_constant<kind>) for a run-time op kind (uniform over a launch), so each
 // kind compiles to its own branch-free body.
 template <typename F>

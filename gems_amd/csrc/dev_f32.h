@@ -18,13 +18,6 @@
 namespace plk {
 namespace dev {
 
-// Renormalisation exponent of a pattern maximum: m * 2^-e in [0.5, 1).
-__device__ __forceinline__ int norm_exponent(float m) {
-    int e = 0;
-    if (m > 0.0f) (void)frexpf(m, &e);
-    return e;
-}
-
 // ---------------------------------------------------------------------------------------
 // P(t) in FP32 form: P_c[x][y] (FP64 math) written as MFMA A fragments
 // frag[c][xb][ks][lane] = P_c[xb*16 + lane%16][ks*4 + lane/16] (zero padded), or plainly
@@ -131,8 +124,22 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             }
         }
         float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride + cbase;
+        // children's exponents, fetched before the matrix work so that their latency
+        // overlaps it (only the wavefronts of category 0 keep exponents)
+        int ech[NPB];
 #pragma unroll
-        for (int g = pb0; g < pb0 + NPB; g += PBG) {
+        for (int j = 0; j < NPB; j++) {
+            ech[j] = 0;
+            if (cw == 0 && lg == 0) {
+                const int64_t pj = (int64_t)tile * 64 + (pb0 + j) * 16 + l16;
+                if (KIND == KIND_TI) ech[j] = expo[(int64_t)op.right * P.scale_stride + pj];
+                else if (KIND == KIND_II)
+                    ech[j] = expo[(int64_t)op.left * P.scale_stride + pj] + expo[(int64_t)op.right * P.scale_stride + pj];
+            }
+        }
+#pragma unroll
+        for (int gi = 0; gi < NPB; gi += PBG) {
+            const int g = pb0 + gi;
             floatx4 out[PBG][XB];
 #pragma unroll
             for (int j = 0; j < PBG; j++) {
@@ -213,20 +220,19 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                         const int x = xb * 16 + lg * 4 + r;
                         if (x < N && active) outp[x * 64 + pl] = ldexpf(out[j][xb][r], -e);
                     }
-                if (cw == 0 && lg == 0) {  // one wave per pattern block keeps the exponents
-                    const int64_t p = (int64_t)tile * 64 + pl;
-                    int es = e;
-                    if (KIND == KIND_TI) es += expo[(int64_t)op.right * P.scale_stride + p];
-                    else if (KIND == KIND_II)
-                        es += expo[(int64_t)op.left * P.scale_stride + p] + expo[(int64_t)op.right * P.scale_stride + p];
-                    expo[(int64_t)op.out_slot * P.scale_stride + p] = es;
-                }
+                if (cw == 0 && lg == 0)  // one wave per pattern block keeps the exponents
+                    expo[(int64_t)op.out_slot * P.scale_stride + (int64_t)tile * 64 + pl] = e + ech[gi + j];
             }
             sp ^= 1;
         }
         });
-        // the next op reads these values from other lanes of this wavefront
-        __syncthreads();
+        // The next op reads these values (and exponents) in other lanes of this same
+        // wavefront: a wavefront's vector memory operations are performed in order, and
+        // the wavefront-scope fences keep the compiler from moving the next op's loads
+        // above these stores.  No workgroup barrier, so a wave does not wait here for
+        // its stores to complete; s_max is double-buffered across the per-op exchange.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (MODE == MODE_PARTIALS) return;
     __syncthreads();  // s_pi visible even when the op list is empty
@@ -594,8 +600,9 @@ __device__ inline void traverse_f32_valu_body(const KParams &P, const int tile) 
 // (sum_y Uinv[i][y] B_c[y]), the eigen-space product the derivative kernels consume
 // (the reference's theta_all, tree/phylokernelnew.h:2187-2193), followed by the first
 // derivative evaluation.  buffer_scale_all = (e_A + e_B) ln 2.
-template <int N>
-__device__ inline void theta_f32_body(const KParams &P, const int tile) {
+// L3: also reduce the logL at the same length (third value), for the many-branch Newton.
+template <int N, bool L3 = false>
+__device__ inline bool theta_f32_body(const KParams &P, const int tile) {
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
     const int ncw = nwaves(), ncat = P.ncat;
@@ -612,9 +619,10 @@ __device__ inline void theta_f32_body(const KParams &P, const int tile) {
         const int c = idx / N, i = idx % N;
         const double cof = P.eval[i] * P.rates[c];
         const double val = exp(cof * P.br_len) * P.props[c];
+        const double val1 = cof * val;  // the same products as fill_val (theta kernels)
         s_val[0][idx] = val;
-        s_val[1][idx] = cof * val;
-        s_val[2][idx] = cof * cof * val;
+        s_val[1][idx] = val1;
+        s_val[2][idx] = cof * val1;
     }
     __syncthreads();
     for (int c = w; c < ncat; c += ncw) {
@@ -646,7 +654,7 @@ __device__ inline void theta_f32_body(const KParams &P, const int tile) {
         s_cat[2][c][lane] = l2;
     }
     __syncthreads();
-    double r0 = 0.0, r1 = 0.0;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     if (w == 0) {
         const int *expo = (const int *)P.scale;
         const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
@@ -664,8 +672,10 @@ __device__ inline void theta_f32_body(const KParams &P, const int tile) {
         const double t1 = dfr * f, t2 = ddfr * f;
         r0 = wave_sum(p < P.nptn ? t1 : 0.0);
         r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
+        if (L3) r2 = wave_sum(p < P.nptn ? (log(lh) + e * kLn2) * f : 0.0);
     }
-    finish_tiles(P, r0, r1, tile, P.ntiles);
+    if constexpr (L3) return finish_tiles3(P, r0, r1, r2, tile, P.ntiles);
+    return finish_tiles(P, r0, r1, tile, P.ntiles);
 }
 
 }  // namespace dev

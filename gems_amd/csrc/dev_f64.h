@@ -194,8 +194,11 @@ __device__ __forceinline__ void fill_val(const KParams &P, double len, double *v
 // SAFE per-category factor relative to the minimum scale (2918-2944): 1, 2^-256 or 0.
 __device__ __forceinline__ double safe_factor(int d) { return d == 0 ? 1.0 : (d == 1 ? kST : 0.0); }
 
-template <int N, int MODE>
-__device__ inline void traverse_f64_body(const KParams &P, const int tile) {
+// L3 (with MODE_THETA): also reduce the logL at the same length as a third value (the
+// value computeLikelihoodFromBuffer would return there), for the many-branch Newton.
+// Returns true in the workgroup that wrote the result.
+template <int N, int MODE, bool L3 = false>
+__device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
     const int ncw = nwaves(), ncat = P.ncat;
@@ -218,17 +221,29 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
         int sl = 0, sr = 0;
         if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
         if (op.kind == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
+        // children's NORM scale counts, fetched before the partial work so that their
+        // latency overlaps it (wave 0 keeps the per-pattern count)
+        unsigned sc_norm = 0;
+        if (!P.safe && w == 0) {
+            if (op.kind == KIND_TI) sc_norm = scale[(int64_t)op.right * P.scale_stride + p];
+            else if (op.kind == KIND_II)
+                sc_norm = (unsigned)scale[(int64_t)op.left * P.scale_stride + p] + scale[(int64_t)op.right * P.scale_stride + p];
+        }
         double wmax = 0.0;
         for (int c = w; c < ncat; c += ncw) {
             const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
             if constexpr (N <= 20) {
-                const double m = op_partial_store<N>(P, op, M, c, off, sl, sr, outp + off);
+                // SAFE: this category's children counts, also fetched ahead of the work
+                unsigned sc = 0;
                 if (P.safe) {
                     const int64_t a = p * ncat + c;
-                    unsigned sc = 0;
                     if (op.kind == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + a];
                     else if (op.kind == KIND_II)
                         sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + a] + scale[(int64_t)op.right * P.scale_stride + a];
+                }
+                const double m = op_partial_store<N>(P, op, M, c, off, sl, sr, outp + off);
+                if (P.safe) {
+                    const int64_t a = p * ncat + c;
                     if (op.kind != KIND_TT && m < kST && invar == 0.0) {
 #pragma unroll
                         for (int j = 0; j < N; j++) outp[off + j * 64] *= kTwo256;
@@ -276,16 +291,10 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
                     for (int j = 0; j < N; j++) outp[off + j * 64] *= kTwo256;
                 }
             }
-            if (w == 0) {
-                unsigned sc = 0;
-                if (op.kind == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + p];
-                else if (op.kind == KIND_II)
-                    sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + p] + scale[(int64_t)op.right * P.scale_stride + p];
-                scale[(int64_t)op.out_slot * P.scale_stride + p] = (uint16_t)(sc + (do_scale ? 1u : 0u));
-            }
+            if (w == 0) scale[(int64_t)op.out_slot * P.scale_stride + p] = (uint16_t)(sc_norm + (do_scale ? 1u : 0u));
         }
     }
-    if (MODE == MODE_PARTIALS) return;
+    if (MODE == MODE_PARTIALS) return false;
 
     // -------- branch stage: node refs a (slot or tip) and b (slot)
     __syncthreads();  // s_val complete
@@ -349,7 +358,7 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
             }
         }
     }
-    double r0 = 0.0, r1 = 0.0;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     if (w == 0) {
         double sc;
         double lh = 0.0, d1 = 0.0, d2 = 0.0;
@@ -388,9 +397,11 @@ __device__ inline void traverse_f64_body(const KParams &P, const int tile) {
             const double t1 = dfr * f, t2 = ddfr * f;
             r0 = wave_sum(p < P.nptn ? t1 : 0.0);
             r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
+            if (L3) r2 = wave_sum(p < P.nptn ? (log(lh) + sc * kLogST) * f : 0.0);
         }
     }
-    finish_tiles(P, r0, r1, tile, P.ntiles);
+    if constexpr (L3) return finish_tiles3(P, r0, r1, r2, tile, P.ntiles);
+    return finish_tiles(P, r0, r1, tile, P.ntiles);
 }
 
 // Newton steps and logL from the resident theta buffer: derivatives
@@ -533,6 +544,20 @@ __device__ inline void newton_after_eval(NewtonState &s, double d1, double d2) {
         s.out = s.rts_old;
         s.done = 1;
     }
+}
+
+// Many-branch form: record the logL at the evaluated length, then take the step (or,
+// for an eval_only request, just record and stop).
+__device__ inline void newton_record(NewtonState &s, double d1, double d2, double logl) {
+    s.logl_eval = logl;
+    s.x_eval = s.rts;
+    if (s.eval_only) {
+        s.eval_only = 0;
+        s.done = 1;
+        return;
+    }
+    if (s.evals == 0) s.logl0 = logl;
+    newton_after_eval(s, d1, d2);
 }
 
 }  // namespace dev

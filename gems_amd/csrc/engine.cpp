@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -65,7 +66,8 @@ int ensure_op_capacity(Ctx &c, int nops) {
     c.d_jobs = nullptr;
     c.d_mats = nullptr;
     c.ops_cap = c.mats_cap = 0;
-    HIP_TRY(hipMallocAsync((void **)&c.d_ops, sizeof(DevOp) * cap, c.stream));
+    // ops, then (level-parallel kernel) nops + 1 level bounds and up to ntaxa + 1 tip ids
+    HIP_TRY(hipMallocAsync((void **)&c.d_ops, sizeof(DevOp) * cap + sizeof(int32_t) * (cap + c.ntaxa + 4), c.stream));
     HIP_TRY(hipMallocAsync((void **)&c.d_jobs, sizeof(PJob) * 2 * cap, c.stream));
     HIP_TRY(hipMallocAsync(&c.d_mats, c.real_size() * c.mat_stride * cap, c.stream));
     HIP_TRY(hipStreamSynchronize(c.stream));  // usable from any stream (batched launches)
@@ -133,6 +135,135 @@ static void timing_collect(Ctx &c, const bool used[3]) {
     }
 }
 
+// Level-parallel 4-state kernel plan: sort the queued ops into dependency levels
+// (read-after-write, write-after-read and write-after-write on slots, so any op list
+// keeps its meaning), give partials consumed later in the launch an LDS buffer (by
+// liveness, at most max_buf), and stage the tips' states.  Reorders c.pending and
+// remaps the P(t) jobs.
+struct SmallPlan {
+    std::vector<int32_t> lvl, tips;
+    int nbuf = 0, br_lds_a = -1, br_lds_b = -1;
+    size_t lds_bytes = 0;
+};
+
+static void plan_small(Ctx &c, bool branch, int br_a, int br_b, SmallPlan &sp) {
+    std::vector<DevOp> &ops = c.pending;
+    const int n = (int)ops.size();
+    auto internal_children = [](const DevOp &o, int *ch) {
+        int m = 0;
+        if (o.kind == KIND_TI) ch[m++] = o.right;
+        else if (o.kind == KIND_II) {
+            ch[m++] = o.left;
+            ch[m++] = o.right;
+        }
+        return m;
+    };
+    // 1. levels
+    std::vector<int> level(n, 0), writer(c.nslots, -1), maxread(c.nslots, -1);
+    int nlev = 0;
+    for (int k = 0; k < n; k++) {
+        int ch[2], m = internal_children(ops[k], ch), lv = 0;
+        for (int j = 0; j < m; j++)
+            if (writer[ch[j]] >= 0) lv = std::max(lv, level[writer[ch[j]]] + 1);
+        const int o = ops[k].out_slot;
+        if (writer[o] >= 0) lv = std::max(lv, level[writer[o]] + 1);
+        lv = std::max(lv, maxread[o] + 1);
+        level[k] = lv;
+        for (int j = 0; j < m; j++) maxread[ch[j]] = std::max(maxread[ch[j]], lv);
+        writer[o] = k;
+        maxread[o] = -1;
+        nlev = std::max(nlev, lv + 1);
+    }
+    // 2. stable sort by level, remap the P(t) jobs
+    std::vector<int> perm(n), newidx(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return level[a] < level[b]; });
+    std::vector<DevOp> sorted(n);
+    for (int i = 0; i < n; i++) {
+        sorted[i] = ops[perm[i]];
+        sorted[i].level = level[perm[i]];
+        newidx[perm[i]] = i;
+    }
+    ops.swap(sorted);
+    for (PJob &j : c.jobs)
+        if (j.op < n) j.op = newidx[j.op];
+    sp.lvl.assign(nlev + 1, 0);
+    for (int i = 0; i < n; i++) sp.lvl[ops[i].level + 1] = i + 1;
+    for (int l = 1; l <= nlev; l++) sp.lvl[l] = std::max(sp.lvl[l], sp.lvl[l - 1]);
+    // 3. producers of every read and the last level that reads each value
+    std::vector<int> current(c.nslots, -1), last_use(n, -1), prod_l(n, -1), prod_r(n, -1);
+    for (int i = 0; i < n; i++) {
+        DevOp &o = ops[i];
+        if (o.kind == KIND_II) {
+            prod_l[i] = current[o.left];
+            if (prod_l[i] >= 0) last_use[prod_l[i]] = std::max(last_use[prod_l[i]], o.level);
+        }
+        if (o.kind != KIND_TT) {
+            prod_r[i] = current[o.right];
+            if (prod_r[i] >= 0) last_use[prod_r[i]] = std::max(last_use[prod_r[i]], o.level);
+        }
+        current[o.out_slot] = i;
+    }
+    int pa = -1, pb = -1;
+    if (branch) {
+        if (br_a >= 0) pa = current[br_a];
+        if (br_b >= 0) pb = current[br_b];
+        if (pa >= 0) last_use[pa] = nlev;
+        if (pb >= 0) last_use[pb] = nlev;
+    }
+    // 4. staged tips (up to 256)
+    std::vector<int> tip_index(c.ntaxa, -1);
+    auto stage_tip = [&](int taxon) {
+        if (tip_index[taxon] < 0 && (int)sp.tips.size() < 256) {
+            tip_index[taxon] = (int)sp.tips.size();
+            sp.tips.push_back(taxon);
+        }
+        return tip_index[taxon];
+    };
+    for (DevOp &o : ops) {
+        o.lds_left = o.lds_right = o.lds_out = -1;
+        if (o.kind <= KIND_TI) o.lds_left = stage_tip(o.left);
+        if (o.kind == KIND_TT) o.lds_right = stage_tip(o.right);
+    }
+    if (branch && br_a < 0) sp.br_lds_a = stage_tip(-br_a - 1);
+    // 5. LDS buffers by liveness within the budget
+    const int form = c.prec == PLK_FP64 ? 0 : 1;
+    const size_t budget = 48 * 1024;  // dynamic LDS; the kernel's own arrays take 14 KB
+    const size_t fixed = small_lds_bytes(form, c.ncat, c.safe, c.su, 0, (int)sp.tips.size());
+    const size_t per_buf = small_lds_bytes(form, c.ncat, c.safe, c.su, 1, 0) - small_lds_bytes(form, c.ncat, c.safe, c.su, 0, 0);
+    const int max_buf = budget > fixed ? (int)std::min<size_t>(64, (budget - fixed) / per_buf) : 0;
+    std::vector<int> buf_of(n, -1), holder;  // holder[b] = value in buffer b
+    for (int l = 0, i = 0; l < nlev; l++) {
+        for (size_t b = 0; b < holder.size(); b++)
+            if (holder[b] >= 0 && last_use[holder[b]] < l) holder[b] = -1;
+        for (; i < n && ops[i].level == l; i++) {
+            if (last_use[i] < 0) continue;
+            int b = -1;
+            for (size_t q = 0; q < holder.size(); q++)
+                if (holder[q] < 0) {
+                    b = (int)q;
+                    break;
+                }
+            if (b < 0 && (int)holder.size() < max_buf) {
+                b = (int)holder.size();
+                holder.push_back(-1);
+            }
+            if (b < 0) continue;
+            holder[b] = i;
+            buf_of[i] = b;
+            ops[i].lds_out = b;
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        if (prod_l[i] >= 0) ops[i].lds_left = buf_of[prod_l[i]];
+        if (prod_r[i] >= 0) ops[i].lds_right = buf_of[prod_r[i]];
+    }
+    if (pa >= 0) sp.br_lds_a = buf_of[pa];
+    if (pb >= 0) sp.br_lds_b = buf_of[pb];
+    sp.nbuf = (int)holder.size();
+    sp.lds_bytes = small_lds_bytes(form, c.ncat, c.safe, c.su, sp.nbuf, (int)sp.tips.size());
+}
+
 // Upload the queued ops, run the P(t) kernel, then one traversal launch in `mode`.
 // The FP32 form also needs P(t) of the branch itself (an extra job after the ops) for
 // the logL stage, and computes theta in a separate FP64 kernel after the traversal.
@@ -145,21 +276,59 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     p.ops = c.d_ops;
     p.mats = c.d_mats;
     p.nops = nops;
+    // 4-state models: level-parallel kernel (partials, and the logL stage when asked)
+    // auto: the level kernel forms P(t) itself, which saves the P(t) launch (what dominates
+    // small alignments) but repeats the exponentials in every tile (which costs on large ones)
+    const int slots = c.small_waves >= 0 ? c.small_waves : (c.ntiles <= 320 ? 4 : 0);
+    const bool small = c.n == 4 && slots > 0 && (nops > 0 || mode == MODE_LOGL);
+    SmallPlan sp;
+    if (small) plan_small(c, mode != MODE_PARTIALS, p.br_a, p.br_b, sp);
+    if (small) c.jobs.clear();  // the level-parallel kernel forms P(t) itself
     const int njobs = (int)c.jobs.size();
-    if (njobs > 0) {
+    if (njobs > 0 || small) {
         size_t ob = sizeof(DevOp) * nops, jb = sizeof(PJob) * njobs;
-        rc = alloc_stage(c, ob + jb);
+        size_t xb = small ? sizeof(int32_t) * (sp.lvl.size() + sp.tips.size()) : 0;
+        rc = alloc_stage(c, ob + xb + jb);
         if (rc) return rc;
-        if (nops > 0) std::memcpy(c.h_stage, c.pending.data(), ob);
-        std::memcpy((char *)c.h_stage + ob, c.jobs.data(), jb);
-        if (nops > 0) HIP_TRY(hipMemcpyAsync(c.d_ops, c.h_stage, ob, hipMemcpyHostToDevice, c.stream));
-        HIP_TRY(hipMemcpyAsync(c.d_jobs, (char *)c.h_stage + ob, jb, hipMemcpyHostToDevice, c.stream));
-        timing_begin(c, 1);
-        HIP_TRY(f32 ? f32_pmatrix(p, c.d_jobs, njobs, c.stream) : f64_pmatrix(p, c.d_jobs, njobs, c.stream));
-        timing_end(c, 1);
-        used[1] = true;
+        char *h = (char *)c.h_stage;
+        if (nops > 0) std::memcpy(h, c.pending.data(), ob);
+        if (small) {
+            std::memcpy(h + ob, sp.lvl.data(), sizeof(int32_t) * sp.lvl.size());
+            if (!sp.tips.empty()) std::memcpy(h + ob + sizeof(int32_t) * sp.lvl.size(), sp.tips.data(), sizeof(int32_t) * sp.tips.size());
+            p.lvl = (const int32_t *)((char *)c.d_ops + ob);
+            p.tips = p.lvl + sp.lvl.size();
+            p.nlevels = (int)sp.lvl.size() - 1;
+            p.ntip = (int)sp.tips.size();
+            p.nbuf = sp.nbuf;
+            p.br_lds_a = sp.br_lds_a;
+            p.br_lds_b = sp.br_lds_b;
+        }
+        if (njobs > 0) std::memcpy(h + ob + xb, c.jobs.data(), jb);
+        if (ob + xb > 0) HIP_TRY(hipMemcpyAsync(c.d_ops, h, ob + xb, hipMemcpyHostToDevice, c.stream));
+        if (njobs > 0) {
+            HIP_TRY(hipMemcpyAsync(c.d_jobs, h + ob + xb, jb, hipMemcpyHostToDevice, c.stream));
+            timing_begin(c, 1);
+            HIP_TRY(f32 ? f32_pmatrix(p, c.d_jobs, njobs, c.stream) : f64_pmatrix(p, c.d_jobs, njobs, c.stream));
+            timing_end(c, 1);
+            used[1] = true;
+        }
     }
-    if (f32) {
+    if (small) {
+        const int smode = mode == MODE_LOGL ? MODE_LOGL : MODE_PARTIALS;
+        timing_begin(c, 0);
+        HIP_TRY(small_traverse(p, f32 ? 1 : 0, smode, slots, sp.lds_bytes, c.stream));
+        timing_end(c, 0);
+        used[0] = true;
+        if (mode == MODE_THETA) {
+            // theta and derivatives at the branch from the partials just computed
+            KParams q = p;
+            q.nops = 0;
+            timing_begin(c, 2);
+            HIP_TRY(f32 ? f32_theta(q, c.stream) : f64_traverse(q, MODE_THETA, c.stream));
+            timing_end(c, 2);
+            used[2] = true;
+        }
+    } else if (f32) {
         const int tmode = mode == MODE_THETA ? MODE_PARTIALS : mode;
         if (nops > 0 || tmode != MODE_PARTIALS) {
             timing_begin(c, 0);
@@ -324,13 +493,14 @@ extern "C" int plk_destroy(plk_ctx *c) {
     void *bufs[] = {c->d_part, c->d_scale, c->d_states, c->d_freq, c->d_invar, c->d_mats, c->d_model,
                     c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
                     c->d_result, c->d_counter, c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres,
-                    c->d_mcount, c->d_newton};
+                    c->d_mcount, c->d_newton, c->d_mnewton};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, c->stream);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->h_result) (void)hipHostFree(c->h_result);
     if (c->h_newton) (void)hipHostFree(c->h_newton);
+    if (c->h_mnewton) (void)hipHostFree(c->h_mnewton);
     for (int i = 0; i < 6; i++)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -444,6 +614,8 @@ extern "C" int plk_update_partials(plk_ctx *c, const plk_op *ops, int nops) {
             d.left = o.child[0];
             d.right = o.child[1];
         }
+        d.len_left = side_of0 == 0 ? o.len[0] : o.len[1];
+        d.len_right = side_of0 == 0 ? o.len[1] : o.len[0];
         c->pending.push_back(d);
         c->jobs.push_back(PJob{k, side_of0, t0 ? 1 : 0, 0, o.len[0]});
         c->jobs.push_back(PJob{k, side_of1, t1 ? 1 : 0, 0, o.len[1]});
@@ -630,6 +802,140 @@ extern "C" int plk_newton_branch(plk_ctx *c, int a, int b, double x1, double xgu
     }
 }
 
+// Buffers of the many-branch launches: per-branch parameters, (branch, tile) items,
+// 3 reduction values per tile and per branch, counters and Newton states.
+static int ensure_multi(Ctx &c, int nbr) {
+    if (nbr <= c.mcap) return PLK_OK;
+    void *old[] = {c.d_mparams, c.d_mitems, c.d_mtile, c.d_mres, c.d_mcount, c.d_mnewton};
+    for (void *q : old)
+        if (q) (void)hipFreeAsync(q, c.stream);
+    if (c.h_mnewton) (void)hipHostFree(c.h_mnewton);
+    c.d_mparams = c.d_mitems = nullptr;
+    c.d_mtile = c.d_mres = nullptr;
+    c.d_mcount = nullptr;
+    c.d_mnewton = c.h_mnewton = nullptr;
+    c.mcap = 0;
+    const int cap = std::max(nbr, 2 * c.ntaxa);
+    HIP_TRY(hipMallocAsync(&c.d_mparams, sizeof(KParams) * cap, c.stream));
+    HIP_TRY(hipMallocAsync(&c.d_mitems, sizeof(BatchItem) * (size_t)cap * c.ntiles, c.stream));
+    HIP_TRY(hipMallocAsync((void **)&c.d_mtile, sizeof(double) * 3 * (size_t)cap * c.ntiles, c.stream));
+    HIP_TRY(hipMallocAsync((void **)&c.d_mres, sizeof(double) * 3 * cap, c.stream));
+    HIP_TRY(hipMallocAsync((void **)&c.d_mcount, sizeof(unsigned) * cap, c.stream));
+    HIP_TRY(hipMallocAsync((void **)&c.d_mnewton, sizeof(NewtonState) * cap, c.stream));
+    HIP_TRY(hipHostMalloc((void **)&c.h_mnewton, sizeof(NewtonState) * cap, hipHostMallocDefault));
+    HIP_TRY(hipMemsetAsync(c.d_mcount, 0, sizeof(unsigned) * cap, c.stream));
+    std::vector<BatchItem> items((size_t)cap * c.ntiles);
+    for (int i = 0; i < cap; i++)
+        for (int tt = 0; tt < c.ntiles; tt++) items[(size_t)i * c.ntiles + tt] = BatchItem{i, tt};
+    HIP_TRY(hipMemcpyAsync(c.d_mitems, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice, c.stream));
+    HIP_TRY(hipStreamSynchronize(c.stream));
+    c.mcap = cap;
+    return PLK_OK;
+}
+
+extern "C" int plk_newton_multi(plk_ctx *c, int nbr, const int *a, const int *b, double x1, const double *xguess,
+                                double x2, double xacc, int max_nr, double *out, double *logl, int *status) {
+    clear_error();
+    int rc = check_ctx(c);
+    if (rc) return rc;
+    if (nbr < 1 || !a || !b || !xguess || !out || !std::isfinite(x1) || !std::isfinite(x2) || x1 < 0 || x2 < x1 ||
+        !(xacc > 0) || max_nr < -1)
+        return fail(PLK_ERR_ARG, "plk_newton_multi: bad arguments");
+    const bool eval_only = max_nr == -1;
+    std::vector<int> ra(a, a + nbr), rb(b, b + nbr);
+    for (int i = 0; i < nbr; i++) {
+        rc = branch_refs(*c, ra[i], rb[i]);
+        if (rc) return rc;
+        if (!std::isfinite(xguess[i])) return fail(PLK_ERR_ARG, "plk_newton_multi: bad initial length");
+    }
+    rc = plk_flush(c);
+    if (rc) return rc;
+    rc = ensure_multi(*c, nbr);
+    if (rc) return rc;
+    KParams base = make_params(*c);
+    base.nops = 0;
+    base.theta = nullptr;
+    base.buf_scale = nullptr;
+    std::vector<KParams> ps(nbr, base);
+    NewtonState *h = c->h_mnewton;
+    for (int i = 0; i < nbr; i++) {
+        ps[i].br_a = ra[i];
+        ps[i].br_b = rb[i];
+        ps[i].tile_sums = c->d_mtile + (size_t)3 * c->ntiles * i;
+        ps[i].counter = c->d_mcount + i;
+        ps[i].result = c->d_mres + 3 * i;
+        h[i] = NewtonState{};
+        h[i].x1 = x1;
+        h[i].x2 = x2;
+        h[i].xacc = xacc;
+        h[i].rts = std::min(std::max(xguess[i], x1), x2);
+        h[i].max_nr = std::max(max_nr, 0);
+        h[i].safe = c->safe;
+        h[i].eval_only = eval_only ? 1 : 0;
+    }
+    rc = alloc_stage(*c, sizeof(KParams) * nbr);
+    if (rc) return rc;
+    std::memcpy(c->h_stage, ps.data(), sizeof(KParams) * nbr);
+    HIP_TRY(hipMemcpyAsync(c->d_mparams, c->h_stage, sizeof(KParams) * nbr, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_mnewton, h, sizeof(NewtonState) * nbr, hipMemcpyHostToDevice, c->stream));
+    const int ncw = std::min(c->ncat, 4);
+    auto run_rounds = [&](int limit) -> int {
+        // evaluation 0 plus up to max_nr iterations; rounds grow until every branch is done
+        int round = std::min(6, std::max(limit, 1)), launched = 0;
+        for (;;) {
+            for (int k = 0; k < round; k++)
+                HIP_TRY(multi_newton(c->prec, ncw, (const KParams *)c->d_mparams, (const BatchItem *)c->d_mitems,
+                                     nbr * c->ntiles, c->d_mnewton, c->stream));
+            launched += round;
+            HIP_TRY(hipMemcpyAsync(h, c->d_mnewton, sizeof(NewtonState) * nbr, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            bool all = true;
+            for (int i = 0; i < nbr && all; i++) all = h[i].done != 0;
+            if (all) return PLK_OK;
+            if (launched > limit + 1) return fail(PLK_ERR_STATE, "plk_newton_multi: solves did not finish");
+            round = std::min(2 * round, 48);
+        }
+    };
+    rc = run_rounds(eval_only ? 1 : max_nr);
+    if (rc) return rc;
+    if (eval_only) {
+        for (int i = 0; i < nbr; i++) {
+            out[i] = h[i].rts;
+            if (logl) logl[i] = h[i].logl_eval;
+            if (status) status[i] = PLK_OK;
+        }
+        c->theta_valid = false;
+        return PLK_OK;
+    }
+    // the logL at the answer is the last evaluation's unless bisection ended on an older
+    // bracket point: evaluate those once more
+    int more = 0;
+    for (int i = 0; i < nbr; i++)
+        if (h[i].status == NEWTON_OK && h[i].out != h[i].x_eval) {
+            h[i].rts = h[i].out;
+            h[i].eval_only = 1;
+            h[i].done = 0;
+            more++;
+        }
+    if (more) {
+        HIP_TRY(hipMemcpyAsync(c->d_mnewton, h, sizeof(NewtonState) * nbr, hipMemcpyHostToDevice, c->stream));
+        rc = run_rounds(1);
+        if (rc) return rc;
+    }
+    int first_err = PLK_OK;
+    for (int i = 0; i < nbr; i++) {
+        int st = h[i].status == NEWTON_OK ? PLK_OK : PLK_ERR_NUMERIC;
+        out[i] = st == PLK_OK ? h[i].out : h[i].rts;
+        if (logl) logl[i] = h[i].logl_eval;
+        if (status) status[i] = st;
+        if (st != PLK_OK && first_err == PLK_OK) first_err = st;
+    }
+    c->theta_valid = false;
+    if (first_err != PLK_OK && !status)
+        return fail(first_err, "plk_newton_multi: a solve failed (non-finite derivative or iteration limit)");
+    return PLK_OK;
+}
+
 extern "C" int plk_get_partial(plk_ctx *c, int slot, double *partial, uint16_t *scale) {
     clear_error();
     if (!c || slot < 0 || slot >= c->nslots) return fail(PLK_ERR_ARG, "plk_get_partial: bad slot");
@@ -705,6 +1011,12 @@ extern "C" int plk_set_tile_shape(plk_ctx *c, int waves_per_category) {
     return PLK_OK;
 }
 
+extern "C" int plk_set_level_waves(plk_ctx *c, int waves) {
+    if (!c || waves < -1 || waves > 4) return fail(PLK_ERR_ARG, "plk_set_level_waves: slots must be in [-1, 4]");
+    c->small_waves = waves;
+    return PLK_OK;
+}
+
 // Device-to-device copy probe: measured HBM bandwidth (bytes read + written per second)
 // for roofline context, and a copy of known size for calibrating traffic counters.
 extern "C" int plk_probe_copy(int64_t bytes, int reps, double *gbps) {
@@ -759,28 +1071,8 @@ extern "C" int plk_branch_derivs_multi(plk_ctx *c, int nbr, const int *a, const 
     }
     rc = plk_flush(c);
     if (rc) return rc;
-    if (nbr > c->mcap) {
-        void *old[] = {c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres, c->d_mcount};
-        for (void *q : old)
-            if (q) (void)hipFreeAsync(q, c->stream);
-        c->d_mparams = c->d_mitems = nullptr;
-        c->d_mtile = c->d_mres = nullptr;
-        c->d_mcount = nullptr;
-        c->mcap = 0;
-        const int cap = std::max(nbr, 2 * c->ntaxa);
-        HIP_TRY(hipMallocAsync(&c->d_mparams, sizeof(KParams) * cap, c->stream));
-        HIP_TRY(hipMallocAsync(&c->d_mitems, sizeof(BatchItem) * (size_t)cap * c->ntiles, c->stream));
-        HIP_TRY(hipMallocAsync((void **)&c->d_mtile, sizeof(double) * 2 * (size_t)cap * c->ntiles, c->stream));
-        HIP_TRY(hipMallocAsync((void **)&c->d_mres, sizeof(double) * 2 * cap, c->stream));
-        HIP_TRY(hipMallocAsync((void **)&c->d_mcount, sizeof(unsigned) * cap, c->stream));
-        HIP_TRY(hipMemsetAsync(c->d_mcount, 0, sizeof(unsigned) * cap, c->stream));
-        std::vector<BatchItem> items((size_t)cap * c->ntiles);
-        for (int i = 0; i < cap; i++)
-            for (int tt = 0; tt < c->ntiles; tt++) items[(size_t)i * c->ntiles + tt] = BatchItem{i, tt};
-        HIP_TRY(hipMemcpyAsync(c->d_mitems, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        c->mcap = cap;
-    }
+    rc = ensure_multi(*c, nbr);
+    if (rc) return rc;
     KParams base = make_params(*c);
     base.nops = 0;
     base.theta = nullptr;

@@ -25,6 +25,13 @@ struct DevOp {
     int32_t kind;
     int32_t left;   // taxon (kind 0/1) or slot (kind 2)
     int32_t right;  // taxon (kind 0) or slot (kind 1/2)
+    // level-parallel 4-state kernel: LDS buffer of the output / of an internal child
+    // (-1: HBM only), or for a tip child its index among the tips staged in LDS
+    int32_t lds_out;
+    int32_t lds_left;
+    int32_t lds_right;
+    int32_t level;
+    double len_left, len_right;  // branch lengths to the children (the P(t) inputs)
 };
 
 // One transition-matrix job of the P(t) kernel: op index, child side, leaf flag, length.
@@ -63,6 +70,13 @@ struct KParams {
     double *tile_sums;        // ntiles x 2
     unsigned *counter;
     double *result;           // 2
+    // level-parallel 4-state kernel (ops sorted by dependency level)
+    const int32_t *lvl;       // nlevels + 1 op-index boundaries
+    int nlevels;
+    int nbuf;                 // LDS partial buffers
+    int ntip;                 // tips whose states are staged in LDS
+    const int32_t *tips;      // their taxon ids
+    int br_lds_a, br_lds_b;   // LDS buffer (or staged tip index) of the branch's two sides
 };
 
 // Device-resident state of one Newton branch-length solve (Optimization::minimizeNewton,
@@ -72,7 +86,10 @@ struct KParams {
 struct NewtonState {
     double x1, x2, xacc;                            // bounds and tolerance
     double rts, rts_old, xl, xh, dx, f, df, out;    // iterate, bracket, last step, -d1, -d2
+    double logl_eval, x_eval, logl0;                // many-branch form: logL at the last evaluated
+                                                    // length x_eval, and at evaluation 0
     int32_t max_nr, j, done, status, evals, safe;
+    int32_t eval_only, pad;                         // eval_only: evaluate logL at rts and stop
 };
 enum : int { NEWTON_OK = 0, NEWTON_UNDERFLOW = 1, NEWTON_NONFINITE = 2, NEWTON_MAXITER = 3 };
 
@@ -111,10 +128,14 @@ struct Ctx {
     bool ptn_lh_valid = false;  // d_ptn_lh holds the per-pattern logL of the last logL evaluation
     bool timing = false;
     int wpc = 1;  // FP32 matrix-core tile shape (plk_set_tile_shape)
+    // 4-state models: ops per level run at once by the level-parallel kernel (0: the
+    // category-split kernels; -1: chosen by size, see plk_set_level_waves)
+    int small_waves = -1;
     // many-branch derivative buffers (plk_branch_derivs_multi)
     void *d_mparams = nullptr, *d_mitems = nullptr;
-    double *d_mtile = nullptr, *d_mres = nullptr;
+    double *d_mtile = nullptr, *d_mres = nullptr;  // 3 values per tile / branch
     unsigned *d_mcount = nullptr;
+    NewtonState *d_mnewton = nullptr, *h_mnewton = nullptr;  // many-branch Newton states (pinned mirror)
     int mcap = 0;
     hipEvent_t ev[6] = {};
     double t_total[3] = {0, 0, 0};
@@ -138,6 +159,9 @@ hipError_t f64_theta_newton(const KParams &p, NewtonState *st, hipStream_t s);
 hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s);
 hipError_t f32_traverse(const KParams &p, int mode, hipStream_t s);
 hipError_t f32_theta(const KParams &p, hipStream_t s);
+// level-parallel 4-state traversal (form 0 FP64, 1 FP32), lds_bytes of dynamic LDS
+hipError_t small_traverse(const KParams &p, int form, int mode, int waves, size_t lds_bytes, hipStream_t s);
+size_t small_lds_bytes(int form, int ncat, int safe, int su, int nbuf, int ntip);
 
 // Shared host helpers (engine.cpp).
 KParams make_params(Ctx &c);
@@ -153,4 +177,6 @@ hipError_t batch_pmatrix(int prec, const KParams *params, const PJob *jobs, int 
 hipError_t batch_traverse(int prec, const KParams *params, const BatchItem *items, int nitems, int grid,
                           hipStream_t s);
 hipError_t multi_derivs(int prec, int ncw, const KParams *params, const BatchItem *items, int nitems, hipStream_t s);
+hipError_t multi_newton(int prec, int ncw, const KParams *params, const BatchItem *items, int nitems,
+                        NewtonState *states, hipStream_t s);
 }  // namespace plk

@@ -88,7 +88,55 @@ __global__ void __launch_bounds__(256) k_multi_derivs_f32(const KParams *params,
     }
 }
 
+// Many-branch Newton: item = (branch, tile); one launch = one derivative evaluation of
+// every unfinished branch at its device-held iterate (with the logL there), then that
+// branch's minimizeNewton step in the workgroup finishing its reduction.  A finished
+// branch's items return at once.  (A branch's state is written only by its last
+// workgroup, after all of its items have read it.)
+__global__ void __launch_bounds__(256) k_multi_newton_f64(const KParams *params, const BatchItem *items,
+                                                          NewtonState *states) {
+    const BatchItem it = load_uniform(items + blockIdx.x);
+    NewtonState *st = states + it.part;
+    if (st->done) return;
+    KParams P = load_uniform(params + it.part);
+    P.br_len = st->rts;
+    bool last = false;
+    switch (P.n) {
+        case 4: last = traverse_f64_body<4, MODE_THETA, true>(P, it.tile); break;
+        case 20: last = traverse_f64_body<20, MODE_THETA, true>(P, it.tile); break;
+        case 61: last = traverse_f64_body<61, MODE_THETA, true>(P, it.tile); break;
+        default: break;
+    }
+    if (last && threadIdx.x == 0) newton_record(*st, P.result[0], P.result[1], P.result[2]);
+}
+
+__global__ void __launch_bounds__(256) k_multi_newton_f32(const KParams *params, const BatchItem *items,
+                                                          NewtonState *states) {
+    const BatchItem it = load_uniform(items + blockIdx.x);
+    NewtonState *st = states + it.part;
+    if (st->done) return;
+    KParams P = load_uniform(params + it.part);
+    P.br_len = st->rts;
+    bool last = false;
+    switch (P.n) {
+        case 4: last = theta_f32_body<4, true>(P, it.tile); break;
+        case 20: last = theta_f32_body<20, true>(P, it.tile); break;
+        case 61: last = theta_f32_body<61, true>(P, it.tile); break;
+        default: break;
+    }
+    if (last && threadIdx.x == 0) newton_record(*st, P.result[0], P.result[1], P.result[2]);
+}
+
 }  // namespace
+
+hipError_t multi_newton(int prec, int ncw, const KParams *params, const BatchItem *items, int nitems,
+                        NewtonState *states, hipStream_t s) {
+    if (nitems <= 0) return hipSuccess;
+    if (prec == PLK_FP64)
+        hipLaunchKernelGGL(k_multi_newton_f64, dim3(nitems), dim3(64 * ncw), 0, s, params, items, states);
+    else hipLaunchKernelGGL(k_multi_newton_f32, dim3(nitems), dim3(64 * ncw), 0, s, params, items, states);
+    return hipGetLastError();
+}
 
 hipError_t multi_derivs(int prec, int ncw, const KParams *params, const BatchItem *items, int nitems, hipStream_t s) {
     if (nitems <= 0) return hipSuccess;

@@ -10,9 +10,9 @@ __global__ void __launch_bounds__(256) k_pmatrix_f64(KParams P, const PJob *__re
 }
 
 // Occupancy floor: the 20-state partial/logL kernels are held to 128 VGPRs (4 waves per
-// SIMD, a few bytes of spill), so that e.g. 782 tiles x 4 waves of a 50k-pattern
-// alignment fit the chip in one round instead of spilling 14 workgroups into a second.
-// The theta variant keeps its full register budget (it would spill far more).
+// SIMD, a few bytes of spill), so that e.g. the 770 tiles x 4 waves of a 49k-pattern
+// alignment fit the chip in one round (3 waves per SIMD hold only 768 tiles).  The
+// theta variant keeps its full register budget (it would spill far more).
 template <int N, int MODE>
 constexpr int traverse_min_waves() { return (N == 20 && MODE != MODE_THETA) ? 4 : 1; }
 

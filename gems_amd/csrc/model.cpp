@@ -287,7 +287,10 @@ extern "C" int plk_tip_lh(int n, int seq_type, const double *inv_evec, double *t
 
 // Standard genetic code over codons AAA..TTT (A,C,G,T order), '*' = stop
 // (alignment/alignment.cpp:44 uses the same NCBI table 1 in this order).
-static const char kStandardCode[] = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF";
+namespace plk {
+const char kStandardGeneticCode[65] = "KNKNTTTTRSRSIIMIQHQHPPPPRRRRLLLLEDEDAAAAGGGGVVVV*Y*YSSSS*CWCLFLF";
+}
+static const char *const kStandardCode = plk::kStandardGeneticCode;
 
 extern "C" int plk_codon_exch(double kappa, double omega, double *exch) {
     if (!(kappa > 0.0) || !(omega > 0.0) || !exch) return plk::fail(PLK_ERR_ARG, "plk_codon_exch: bad arguments");

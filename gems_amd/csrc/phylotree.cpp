@@ -28,6 +28,8 @@ class PhyloTreeGPU {
     // Newton branch solves run device-resident (plk_newton_branch) unless switched to
     // the host loop below, which issues one derivative call per iteration.
     bool device_newton = true;
+    // batched NNI: scratch slots start at nni_base, 3 per candidate, nni_cands candidates
+    int nni_base = 0, nni_cands = 0;
 
     bool is_leaf(int v) const { return v < ntaxa; }
 
@@ -284,6 +286,182 @@ class PhyloTreeGPU {
         return PLK_OK;
     }
 
+    // getBestNNIForBran (tree/phylotree.cpp:4176-4429) with NNI5, for many internal
+    // branches at once: both moves of every branch are evaluated in lockstep.  A move
+    // swaps node1's first other neighbour A with node2's move-th other neighbour X (each
+    // keeps its branch length), then optimizes node1's two other branches, the central
+    // branch and node2's two other branches (NNI_MAX_NR_STEP Newton steps each); the
+    // score is the logL at the last branch.  Each step is one traversal launch over all
+    // candidates' new partials (scratch slots) plus one many-branch Newton solve; the
+    // subtrees around a move are the tree's own directional partials.  The tree is not
+    // changed.
+    int nni_evaluate(int nbr, const int *n1s, const int *n2s, double min_bl, double max_bl, int max_nr,
+                     double *score, double *lens) {
+        if (!all_dirs || nni_cands == 0) return fail(PLK_ERR_STATE, "batched NNI needs a tree created with all_directions = 2");
+        const int ncand = 2 * nbr;
+        if (ncand > nni_cands) return fail(PLK_ERR_ARG, "plk_tree_nni_evaluate: too many branches");
+        struct Cand {
+            int n1, n2;
+            int y[2], z[2];        // node1's / node2's other neighbours after the swap, in slot order
+            int yref[2], zref[2];  // their subtrees' node references (tips or directional slots)
+            double ylen[2], zlen[2], clen;
+            int s1, s2, st;        // scratch slots: node1 side seen from node2, node2 side seen from node1, one more
+        };
+        std::vector<Cand> cs(ncand);
+        for (int b = 0; b < nbr; b++) {
+            const int n1 = n1s[b], n2 = n2s[b];
+            if (n1 < ntaxa || n2 < ntaxa || n1 >= nnodes || n2 >= nnodes || dir(n1, n2) < 0)
+                return fail(PLK_ERR_ARG, "plk_tree_nni_evaluate: not an internal branch");
+            // every subtree partial around the branch, oriented toward it
+            ops.clear();
+            for (int k = 0; k < 3; k++) {
+                const int u = nei[n1 * 3 + k], v = nei[n2 * 3 + k];
+                if (u != n2) collect(n1, u);
+                if (v != n1) collect(n2, v);
+            }
+            n_ops += (int64_t)ops.size();
+            if (!ops.empty()) {
+                int rc = plk_update_partials(ctx, ops.data(), (int)ops.size());
+                if (rc) return rc;
+            }
+            int a_slot = -1, xs[2], m = 0;
+            for (int k = 0; k < 3; k++)
+                if (nei[n1 * 3 + k] != n2 && a_slot < 0) a_slot = k;
+            for (int k = 0; k < 3; k++)
+                if (nei[n2 * 3 + k] != n1) xs[m++] = k;
+            for (int mv = 0; mv < 2; mv++) {
+                Cand &c = cs[2 * b + mv];
+                c.n1 = n1;
+                c.n2 = n2;
+                const int x_slot = xs[mv];
+                const int A = nei[n1 * 3 + a_slot], X = nei[n2 * 3 + x_slot];
+                int qi = 0, ri = 0;
+                for (int k = 0; k < 3; k++) {
+                    const int u = nei[n1 * 3 + k];
+                    if (u == n2) continue;
+                    const int yy = (k == a_slot) ? X : u;
+                    c.y[qi] = yy;
+                    c.yref[qi] = (k == a_slot) ? ref_dir(n2, X) : ref_dir(n1, u);
+                    c.ylen[qi] = (k == a_slot) ? len[n2 * 3 + x_slot] : len[n1 * 3 + k];
+                    qi++;
+                }
+                for (int k = 0; k < 3; k++) {
+                    const int v = nei[n2 * 3 + k];
+                    if (v == n1) continue;
+                    const int zz = (k == x_slot) ? A : v;
+                    c.z[ri] = zz;
+                    c.zref[ri] = (k == x_slot) ? ref_dir(n1, A) : ref_dir(n2, v);
+                    c.zlen[ri] = (k == x_slot) ? len[n1 * 3 + a_slot] : len[n2 * 3 + k];
+                    ri++;
+                }
+                c.clen = length(n1, n2);
+                const int base = nni_base + 3 * (2 * b + mv);
+                c.s1 = base;
+                c.s2 = base + 1;
+                c.st = base + 2;
+            }
+        }
+        // partials of the directions the moves read must be current before the scratch ops
+        int rc = plk_flush(ctx);
+        if (rc) return rc;
+        std::vector<int> ra(ncand), rb(ncand), st(ncand);
+        std::vector<double> guess(ncand), out(ncand), logl(ncand);
+        std::vector<plk_op> nops;
+        auto mkop = [](int out_slot, int c0, double l0, int c1, double l1) {
+            plk_op o{};
+            o.out_slot = out_slot;
+            o.child[0] = c0;
+            o.child[1] = c1;
+            o.len[0] = l0;
+            o.len[1] = l1;
+            return o;
+        };
+        // one step: queue ops, then Newton on (a, b) from the current length; returns the
+        // lengths after optimizeOneBranch's divergence check
+        auto step = [&](auto make_ops, auto branch, auto cur) -> int {
+            nops.clear();
+            for (int i = 0; i < ncand; i++) make_ops(cs[i]);
+            if (!nops.empty()) {
+                int r = plk_update_partials(ctx, nops.data(), (int)nops.size());
+                if (r) return r;
+            }
+            for (int i = 0; i < ncand; i++) {
+                branch(cs[i], ra[i], rb[i]);
+                guess[i] = cur(cs[i]);
+            }
+            int r = plk_newton_multi(ctx, ncand, ra.data(), rb.data(), min_bl, guess.data(), max_bl, min_bl, max_nr,
+                                     out.data(), logl.data(), st.data());
+            if (r) return r;
+            for (int i = 0; i < ncand; i++)
+                if (st[i] != PLK_OK) return fail(st[i], "batched NNI: a branch solve failed (non-finite derivative)");
+            n_derivs += ncand;
+            // optimizeOneBranch's divergence check (tree/phylotree.cpp:2642-2651)
+            std::vector<int> div;
+            for (int i = 0; i < ncand; i++)
+                if (out[i] > max_bl * 0.95) div.push_back(i);
+            if (!div.empty()) {
+                std::vector<int> da, db;
+                std::vector<double> dg, dout(div.size()), dl(div.size());
+                std::vector<int> dst(div.size());
+                for (int i : div) {
+                    da.push_back(ra[i]);
+                    db.push_back(rb[i]);
+                    dg.push_back(guess[i]);
+                }
+                // logL at the starting length
+                r = plk_newton_multi(ctx, (int)div.size(), da.data(), db.data(), min_bl, dg.data(), max_bl, min_bl, -1,
+                                     dout.data(), dl.data(), dst.data());
+                if (r) return r;
+                for (size_t q = 0; q < div.size(); q++)
+                    if (dl[q] > logl[div[q]]) {
+                        out[div[q]] = guess[div[q]];
+                        logl[div[q]] = dl[q];
+                    }
+            }
+            return PLK_OK;
+        };
+        // 1. node1's first branch: node2 side seen from node1, node1 side seen from y0
+        rc = step([&](Cand &c) {
+                      nops.push_back(mkop(c.s2, c.zref[0], c.zlen[0], c.zref[1], c.zlen[1]));
+                      nops.push_back(mkop(c.st, c.yref[1], c.ylen[1], c.s2, c.clen));
+                  },
+                  [&](Cand &c, int &a, int &b) { a = c.yref[0]; b = c.st; }, [&](Cand &c) { return c.ylen[0]; });
+        if (rc) return rc;
+        for (int i = 0; i < ncand; i++) cs[i].ylen[0] = out[i];
+        // 2. node1's second branch
+        rc = step([&](Cand &c) { nops.push_back(mkop(c.st, c.yref[0], c.ylen[0], c.s2, c.clen)); },
+                  [&](Cand &c, int &a, int &b) { a = c.yref[1]; b = c.st; }, [&](Cand &c) { return c.ylen[1]; });
+        if (rc) return rc;
+        for (int i = 0; i < ncand; i++) cs[i].ylen[1] = out[i];
+        // 3. the central branch
+        rc = step([&](Cand &c) { nops.push_back(mkop(c.s1, c.yref[0], c.ylen[0], c.yref[1], c.ylen[1])); },
+                  [&](Cand &c, int &a, int &b) { a = c.s1; b = c.s2; }, [&](Cand &c) { return c.clen; });
+        if (rc) return rc;
+        for (int i = 0; i < ncand; i++) cs[i].clen = out[i];
+        // 4. node2's first branch
+        rc = step([&](Cand &c) { nops.push_back(mkop(c.st, c.zref[1], c.zlen[1], c.s1, c.clen)); },
+                  [&](Cand &c, int &a, int &b) { a = c.zref[0]; b = c.st; }, [&](Cand &c) { return c.zlen[0]; });
+        if (rc) return rc;
+        for (int i = 0; i < ncand; i++) cs[i].zlen[0] = out[i];
+        // 5. node2's second branch; the score is the logL there
+        rc = step([&](Cand &c) { nops.push_back(mkop(c.st, c.zref[0], c.zlen[0], c.s1, c.clen)); },
+                  [&](Cand &c, int &a, int &b) { a = c.zref[1]; b = c.st; }, [&](Cand &c) { return c.zlen[1]; });
+        if (rc) return rc;
+        for (int i = 0; i < ncand; i++) {
+            cs[i].zlen[1] = out[i];
+            if (score) score[i] = logl[i];
+            if (lens) {
+                double *l = lens + 5 * i;
+                l[0] = cs[i].clen;
+                l[1] = cs[i].ylen[0];
+                l[2] = cs[i].ylen[1];
+                l[3] = cs[i].zlen[0];
+                l[4] = cs[i].zlen[1];
+            }
+        }
+        return PLK_OK;
+    }
+
     // MTree::findFarthestLeaf (tree/mtree.cpp:3040-3060)
     int farthest_leaf(int node, int dad, std::vector<int> &h) const {
         if (dad >= 0 && is_leaf(node)) {
@@ -373,11 +551,14 @@ extern "C" int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int nta
     t->ntaxa = ntaxa;
     t->nnodes = 2 * ntaxa - 2;
     t->all_dirs = all_directions != 0;
+    t->nni_base = 3 * (ntaxa - 2);
+    t->nni_cands = all_directions == 2 ? 2 * (ntaxa - 3) : 0;
     t->nei.assign((size_t)t->nnodes * 3, -1);
     t->len.assign((size_t)t->nnodes * 3, 0.0);
     t->orient.assign(t->nnodes, -1);
     t->computed.assign(t->all_dirs ? (size_t)3 * (ntaxa - 2) : (size_t)t->nnodes, 0);
-    const int nslots = t->all_dirs ? 3 * (ntaxa - 2) : ntaxa - 2;
+    // all_directions == 2 also reserves 3 scratch slots per NNI candidate (2 per internal branch)
+    const int nslots = t->all_dirs ? 3 * (ntaxa - 2) + 3 * t->nni_cands : ntaxa - 2;
     int rc = plk_create(&t->ctx, nstates, ncat, ntaxa, nptn, nslots, precision, scaling);
     if (rc) {
         delete t;
@@ -493,6 +674,13 @@ extern "C" int plk_tree_get_partial(plk_tree *t, int dad, int node, double *part
         if (rc) return rc;
     }
     return plk_get_partial(t->ctx, t->slot_of(dad, node), partial, scale);
+}
+
+extern "C" int plk_tree_nni_evaluate(plk_tree *t, int nbr, const int32_t *node1, const int32_t *node2, double min_bl,
+                                     double max_bl, int max_nr_step, double *score, double *lens) {
+    if (!t || nbr < 1 || !node1 || !node2 || !(min_bl >= 0) || !(max_bl > min_bl) || max_nr_step < 0)
+        return fail(PLK_ERR_ARG, "plk_tree_nni_evaluate: bad arguments");
+    return t->nni_evaluate(nbr, node1, node2, min_bl, max_bl, max_nr_step, score, lens);
 }
 
 extern "C" int plk_tree_set_newton(plk_tree *t, int device_resident) {

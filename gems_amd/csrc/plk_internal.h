@@ -19,4 +19,7 @@ constexpr double kScalingThreshold = 8.636168555094444625386e-78;  // 2^-256
 constexpr double kLogScalingThreshold = -177.4456782233459932741;  // log(2^-256)
 constexpr double kLn2 = 0.69314718055994530941723212145818;
 
+// NCBI standard genetic code over codons AAA..TTT (A,C,G,T order), '*' = stop.
+extern const char kStandardGeneticCode[65];
+
 }  // namespace plk

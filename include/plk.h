@@ -135,6 +135,17 @@ int plk_loglik_from_theta(plk_ctx *ctx, double t, double *logl);
 int plk_newton_branch(plk_ctx *ctx, int a, int b, double x1, double xguess, double x2, double xacc,
                       int max_nr, double *out, int *evals);
 
+/* nbr independent Newton solves at once (minimizeNewton per branch, as plk_newton_branch,
+ * with bounds x1, x2, tolerance xacc and max_nr shared): every launch evaluates the
+ * derivatives of all unfinished branches, over (branch, tile) work items, from the two
+ * partials (theta is not stored), and the step logic runs on the device.  out[i] is the
+ * minimiser, logl[i] (nullable) the logL at out[i] (computeLikelihoodFromBuffer's value
+ * there), status[i] (nullable) PLK_OK or PLK_ERR_NUMERIC per solve (with status NULL a
+ * failed solve fails the call).  max_nr = -1 solves nothing: logl[i] = the logL at
+ * clamp(xguess[i], x1, x2) (optimizeOneBranch's divergence check needs it). */
+int plk_newton_multi(plk_ctx *ctx, int nbr, const int *a, const int *b, double x1, const double *xguess,
+                     double x2, double xacc, int max_nr, double *out, double *logl, int *status);
+
 /* Read back one slot: partial in canonical order [ptn][c][x] as FP64 (the FP32 form
  * returns its own representation), and its scale counts (NORM: nptn uint16; SAFE:
  * nptn*ncat uint16 [ptn][c]).  Either pointer may be NULL. */
@@ -150,6 +161,15 @@ int plk_get_exponents(plk_ctx *ctx, int slot, int32_t *exponents);
  * parallelism; fewer reuse each matrix fragment over more patterns).  Results do not
  * depend on it. */
 int plk_set_tile_shape(plk_ctx *ctx, int waves_per_category);
+
+/* 4-state models: 1..4 = level-parallel kernel: the ops are sorted into dependency
+ * levels and a 64-pattern tile runs that many independent ops of a level at once
+ * (min(ncat, 4) wavefronts each), keeping partials consumed within the launch in LDS and
+ * forming P(t) itself (no P(t) launch); 0 = one wavefront per category walking the ops
+ * in order after a P(t) launch; -1 (default) = 4 for alignments of at most 320 tiles of
+ * 64 patterns, else 0.  Results do not depend on it (partials and scale counts are
+ * bit-identical in both forms). */
+int plk_set_level_waves(plk_ctx *ctx, int slots);
 
 /* Kernel timing with HIP events recorded on the context's stream around every launch
  * of the partial-traversal kernel.  kind: 0 traversal, 1 P(t), 2 derivative. */

@@ -28,7 +28,9 @@ typedef struct plk_tree plk_tree;
 
 int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling);
 /* all_directions != 0: one slot per direction (3(T-2) slots) instead of LM_PER_NODE's one
- * per internal node, so every branch's two partials coexist (plk_tree_all_branch_derivs). */
+ * per internal node, so every branch's two partials coexist (plk_tree_all_branch_derivs).
+ * all_directions == 2 also reserves the scratch slots of plk_tree_nni_evaluate (6 per
+ * internal branch). */
 int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling,
                        int all_directions);
 int plk_tree_destroy(plk_tree *t);
@@ -56,6 +58,15 @@ int plk_tree_optimize_one_branch(plk_tree *t, int node1, int node2, double min_b
  * plk_newton_branch; 0 runs minimizeNewton on the host with one plk_branch_derivs call per
  * iteration.  Both give the same lengths. */
 int plk_tree_set_newton(plk_tree *t, int device_resident);
+/* NNI evaluation of getBestNNIForBran (tree/phylotree.cpp:4176-4429) with NNI5 for the
+ * internal branches (node1[i], node2[i]), both moves of every branch at once (tree created
+ * with all_directions == 2).  Move m of a branch swaps node1's first other neighbour with
+ * node2's m-th other neighbour; the move's five branches are optimised in the reference's
+ * order with max_nr_step Newton steps (NNI_MAX_NR_STEP).  score[2i + m] = the move's logL,
+ * lens[5(2i + m) + 0..4] = its optimised lengths (central, node1's two, node2's two, as
+ * newLen[0..4]).  The tree is left unchanged. */
+int plk_tree_nni_evaluate(plk_tree *t, int nbr, const int32_t *node1, const int32_t *node2, double min_bl,
+                          double max_bl, int max_nr_step, double *score, double *lens);
 int plk_tree_optimize_all_branches(plk_tree *t, int iterations, double tol, double min_bl, double max_bl,
                                    int max_nr_step, double *logl);
 /* Invalidate every partial and run one full traversal + branch logL (the benchmark step). */

@@ -921,3 +921,60 @@ void orc_rell_save_tree(int nsamples, int nptn, const float *counts, const doubl
     }
     free(lh);
 }
+
+/* One NNI move on internal branch (node1, node2) scored as getBestNNIForBran does with
+ * NNI5 (tree/phylotree.cpp:4176-4429, params.nni5 default true, nni5_num_eval 1):
+ * node1's first neighbour other than node2 (A) is swapped with node2's move-th other
+ * neighbour (X), each keeping its branch length (updateNeighbor keeps the Neighbor
+ * object); then optimizeOneBranch is run on node1's two other branches in neighbour
+ * order, the central branch, and node2's two other branches, with max_nr Newton steps;
+ * the score is the logL from the theta buffer of the last branch (4356-4357).  The tree
+ * is restored afterwards.  lens[5] = new lengths (central, node1's two, node2's two, in
+ * the order of newLen[0..4], 4323-4350). */
+double orc_nni_eval(orc_tree *t, int node1, int node2, int move, double min_bl, double max_bl, int max_nr,
+                    double *lens) {
+    const int nn = t->nnodes;
+    int *nei0 = (int *)malloc(sizeof(int) * 3 * nn);
+    double *len0 = (double *)malloc(sizeof(double) * 3 * nn);
+    memcpy(nei0, t->nei, sizeof(int) * 3 * nn);
+    memcpy(len0, t->len, sizeof(double) * 3 * nn);
+    int a_slot = -1, x_slot = -1, cnt = 0;
+    for (int k = 0; k < 3; k++)
+        if (t->nei[node1 * 3 + k] != node2 && a_slot < 0) a_slot = k;
+    for (int k = 0; k < 3; k++)
+        if (t->nei[node2 * 3 + k] != node1) {
+            if (cnt == move) x_slot = k;
+            cnt++;
+        }
+    const int A = t->nei[node1 * 3 + a_slot], X = t->nei[node2 * 3 + x_slot];
+    const double lenA = t->len[node1 * 3 + a_slot], lenX = t->len[node2 * 3 + x_slot];
+    const int dA = dir_of(t, A, node1), dX = dir_of(t, X, node2);
+    t->nei[node1 * 3 + a_slot] = X;
+    t->len[node1 * 3 + a_slot] = lenX;
+    t->nei[dX] = node1;
+    t->nei[node2 * 3 + x_slot] = A;
+    t->len[node2 * 3 + x_slot] = lenA;
+    t->nei[dA] = node2;
+    orc_clear_all(t);
+    int li = 1;
+    for (int k = 0; k < 3; k++) {
+        int y = t->nei[node1 * 3 + k];
+        if (y == node2) continue;
+        lens[li++] = orc_optimize_one_branch(t, node1, y, min_bl, max_bl, max_nr);
+    }
+    lens[0] = orc_optimize_one_branch(t, node1, node2, min_bl, max_bl, max_nr);
+    int last = -1;
+    for (int k = 0; k < 3; k++) {
+        int z = t->nei[node2 * 3 + k];
+        if (z == node1) continue;
+        lens[li++] = orc_optimize_one_branch(t, node2, z, min_bl, max_bl, max_nr);
+        last = z;
+    }
+    const double score = orc_loglik_from_theta(t, node2, last);
+    memcpy(t->nei, nei0, sizeof(int) * 3 * nn);
+    memcpy(t->len, len0, sizeof(double) * 3 * nn);
+    orc_clear_all(t);
+    free(nei0);
+    free(len0);
+    return score;
+}

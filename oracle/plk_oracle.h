@@ -22,6 +22,7 @@
  *   tip vectors                   tree/phylotreesse.cpp:354-375, model/modelsubst.cpp:183-208,
  *                                 model/modeldna.cpp:462-476, model/modelprotein.cpp:1240-1256
  *   RELL / UFBoot replicate update tree/iqtree.cpp:3852-3931
+ *   NNI5 move evaluation          getBestNNIForBran tree/phylotree.cpp:4176-4429
  *
  * Memory layout follows the reference's pattern-interleaved SIMD layout with
  * ORC_V lanes: element (ptn, c, x) of a partial sits at double offset
@@ -37,6 +38,8 @@
  *   tip vectors      closed form Uinv * indicator(state set)
  *   RELL             counts = pattern frequencies reproduce the tree logL (to the float
  *                    rounding of pattern_lh); the update rule on a hand-worked sequence
+ *   NNI              the move's score equals the logL of the swapped tree, built from
+ *                    scratch, after the same five branch optimisations; the tree is restored
  */
 #ifndef PLK_ORACLE_H
 #define PLK_ORACLE_H
@@ -87,6 +90,10 @@ double orc_optimize_all_branches(orc_tree *t, int iterations, double tol, double
                                  double max_bl, int max_nr_step);
 
 void orc_tip_lh(int nstates, int seq_type, const double *inv_evec, double *out);
+/* NNI5 score of one move on internal branch (node1, node2) (getBestNNIForBran); lens[5]
+ * receives the five optimised lengths (central first).  The tree is restored. */
+double orc_nni_eval(orc_tree *t, int node1, int node2, int move, double min_bl, double max_bl, int max_nr,
+                    double *lens);
 /* UFBoot RELL scoring and replicate update (IQTree::saveCurrentTree); uniform == NULL
  * scores only.  counts: nsamples x nptn. */
 void orc_rell_save_tree(int nsamples, int nptn, const float *counts, const double *ptn_lh, double cur_logl,

@@ -69,6 +69,8 @@ def oracle_lib(lanes=None):
                                                         ctypes.c_double, ctypes.c_int]),
         "orc_tip_lh": (None, [ctypes.c_int, ctypes.c_int, dp, dp]),
         "orc_state_unknown": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+        "orc_nni_eval": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_int, dp]),
         "orc_rell_save_tree": (None, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), dp, ctypes.c_double,
                                       ctypes.c_double, dp, dp, dp, ctypes.POINTER(ctypes.c_int32), dp,
                                       ctypes.POINTER(ctypes.c_int32)]),
@@ -149,6 +151,11 @@ class Oracle:
 
     def optimize_all(self, iterations=100, tol=1e-3, min_bl=1e-6, max_bl=10.0, max_nr=100):
         return self.lib.orc_optimize_all_branches(self.h, iterations, tol, min_bl, max_bl, max_nr)
+
+    def nni_eval(self, n1, n2, move, min_bl=1e-6, max_bl=10.0, max_nr=10):
+        lens = np.zeros(5)
+        sc = self.lib.orc_nni_eval(self.h, n1, n2, move, min_bl, max_bl, max_nr, _dp(lens))
+        return sc, lens
 
     def optimize_one(self, a, b, min_bl=1e-6, max_bl=10.0, max_nr=100):
         return self.lib.orc_optimize_one_branch(self.h, a, b, min_bl, max_bl, max_nr)

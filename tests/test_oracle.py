@@ -215,3 +215,46 @@ def test_rell_update_rule_hand_worked():
     for lh, u, imp, best, cnt, orig in seq:
         _, improved = r.save_tree(np.array([lh]), lh, eps=0.5, uniform=np.array([u]))
         assert improved[0] == imp and r.logl[0] == best and r.best_counts[0] == cnt and r.orig[0] == orig, lh
+
+
+def _swap(nei, blen, n1, n2, move):
+    """The NNI move of getBestNNIForBran on copies of (nei, blen): node1's first other
+    neighbour trades places (slot and branch length) with node2's move-th other one."""
+    nei, blen = nei.copy(), blen.copy()
+    a_slot = [k for k in range(3) if nei[n1, k] != n2][0]
+    x_slot = [k for k in range(3) if nei[n2, k] != n1][move]
+    A, X = int(nei[n1, a_slot]), int(nei[n2, x_slot])
+    la, lx = blen[n1, a_slot], blen[n2, x_slot]
+    ka = [k for k in range(3) if nei[A, k] == n1][0]
+    kx = [k for k in range(3) if nei[X, k] == n2][0]
+    nei[n1, a_slot], blen[n1, a_slot] = X, lx
+    nei[X, kx], blen[X, kx] = n1, lx
+    nei[n2, x_slot], blen[n2, x_slot] = A, la
+    nei[A, ka], blen[A, ka] = n2, la
+    return nei, blen
+
+
+def test_nni_eval_equals_swapped_tree_from_scratch():
+    """orc_nni_eval's score is the logL of the swapped tree, built from scratch, after the
+    same five branch optimisations in the same order; the tree is left as it was."""
+    rng = np.random.default_rng(41)
+    p = S.make_problem(4, 9, 300, rng)
+    o = S.Oracle(p)
+    a, b = 0, int(p.nei[0, 0])
+    before = o.loglik(a, b)
+    T = p.ntaxa
+    checked = 0
+    for n1 in range(T, 2 * T - 2):
+        for n2 in [int(x) for x in p.nei[n1] if x > n1]:
+            for move in (0, 1):
+                score, lens = o.nni_eval(n1, n2, move)
+                nei, blen = _swap(p.nei, p.blen, n1, n2, move)
+                q = S.Problem(**{**p.__dict__, "nei": nei, "blen": blen})
+                f = S.Oracle(q)
+                order = [(n1, int(y)) for y in nei[n1] if y != n2] + [(n1, n2)] + [(n2, int(z)) for z in nei[n2] if z != n1]
+                got = [f.optimize_one(u, v, max_nr=10) for (u, v) in order]
+                assert np.array_equal(np.array([got[2], got[0], got[1], got[3], got[4]]), lens)
+                assert f.loglik_from_theta(*order[-1]) == score
+                checked += 1
+    assert checked >= 8
+    assert o.loglik(a, b) == before
