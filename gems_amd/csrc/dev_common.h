@@ -34,6 +34,40 @@ __device__ __forceinline__ T *opaque(T *p) {
     return p;
 }
 
+// Op descriptors and level bounds are uploaded before a launch and never written during
+// it.  Read through the constant address space they arrive by scalar loads; read as plain
+// global memory (which the kernel also stores to) the compiler uses a vector load, whose
+// wait (vmcnt) also drains every store the wave has in flight, so each op would first
+// wait for the previous op's stores to complete.
+__device__ __forceinline__ int load_int(const int32_t *v, int k) {
+    return ((const __attribute__((address_space(4))) int32_t *)v)[k];
+}
+__device__ __forceinline__ DevOp load_op(const DevOp *ops, int k) {
+    static_assert(sizeof(DevOp) == 48, "8 int32 fields and 2 doubles");
+    const int32_t *w = (const int32_t *)(ops + k);
+    const cdptr d = as_const((const double *)(ops + k));
+    DevOp o;
+    o.out_slot = load_int(w, 0);
+    o.kind = load_int(w, 1);
+    o.left = load_int(w, 2);
+    o.right = load_int(w, 3);
+    o.lds_out = load_int(w, 4);
+    o.lds_left = load_int(w, 5);
+    o.lds_right = load_int(w, 6);
+    o.level = load_int(w, 7);
+    o.len_left = d[4];
+    o.len_right = d[5];
+    return o;
+}
+
+// Op index of step k for a tile: tiles alternate between the op list's own order and the
+// second order in P.perm (when given), by tile parity flipped every 256 tiles, so that both
+// orders run on every XCD and on most compute units.
+__device__ __forceinline__ int op_index(const KParams &P, int tile, int k) {
+    if (P.perm == nullptr || (((tile >> 8) ^ tile) & 1) == 0) return k;
+    return load_int(P.perm, k);
+}
+
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 __device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
 

@@ -87,29 +87,39 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     float *__restrict__ part = (float *)P.part;
     int *__restrict__ expo = (int *)P.scale;
     const int64_t cbase = ((int64_t)tile * ncat + c) * N * 64;  // (tile, c, x=0, lane=0)
-    __shared__ float s_max[2][PLK_GPU_MAX_CATS][64];
+    __shared__ float s_max[2][PLK_GPU_MAX_CATS + 1][64];
     __shared__ double s_cat[PLK_GPU_MAX_CATS][64];
     __shared__ float s_pi[64];
     if (MODE == MODE_LOGL && threadIdx.x < N) s_pi[threadIdx.x] = (float)P.pi[threadIdx.x];
 
-    // pattern blocks per renormalisation group: all four for <= 32 states; the 61-state
-    // model holds its 2 x 64 A fragments in registers, so it works on one block at a time
-    constexpr int PBG = (N > 32) ? 1 : NPB;
+    // pattern blocks per renormalisation group (outputs held in registers across the
+    // category exchange): two for <= 32 states, whose loads for all of the wave's blocks
+    // are issued at the op's start (HOIST); the 61-state model holds its 2 x 64 A
+    // fragments in registers, so it loads and works on one block at a time
+    constexpr bool HOIST = N <= 32;
+    constexpr int PBG = (N > 32) ? 1 : (NPB < 2 ? NPB : 2);
+    constexpr int NB = HOIST ? NPB : PBG;  // pattern blocks whose child columns are held
     int sp = 0;  // double-buffered s_max
     for (int k = 0; k < P.nops; k++) {
-        const DevOp op = P.ops[k];
+        const int ko = op_index(P, tile, k);
+        const DevOp op = load_op(P.ops, ko);
         // the op kind is uniform over the launch: each kind gets its own straight-line
         // code, so all loads of an op are issued together
         with_kind(op.kind, [&](auto KC) {
         constexpr int KIND = decltype(KC)::value;
-        const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
+        const float *__restrict__ M = (const float *)P.mats + (int64_t)ko * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
         const uint8_t *str = P.states + (int64_t)op.right * P.nptn_pad + (int64_t)tile * 64;
         const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
         const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase;
+        // Every global load of an op (A fragments, the children's columns, tip states and
+        // exponents) is issued before the op's first matrix instruction, unconditionally
+        // (inactive waves read category 0 and store nothing; rows past N read row N-1 and
+        // are zeroed by a select), so an op costs one memory round trip (two with a tip
+        // child, whose table rows depend on its states) instead of one per pattern block.
         float aL[XB][KS], aR[XB][KS];
-        if (active && KIND != KIND_TT) {
+        if (KIND != KIND_TT) {
             const float *__restrict__ FRm = M + (int64_t)(ncat + c) * FR;
 #pragma unroll
             for (int xb = 0; xb < XB; xb++)
@@ -124,40 +134,66 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             }
         }
         float *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride + cbase;
-        // children's exponents, fetched before the matrix work so that their latency
-        // overlaps it (only the wavefronts of category 0 keep exponents)
+        // children's exponents (kept by the wavefronts of category 0, lanes 0-15)
         int ech[NPB];
 #pragma unroll
         for (int j = 0; j < NPB; j++) {
-            ech[j] = 0;
-            if (cw == 0 && lg == 0) {
-                const int64_t pj = (int64_t)tile * 64 + (pb0 + j) * 16 + l16;
-                if (KIND == KIND_TI) ech[j] = expo[(int64_t)op.right * P.scale_stride + pj];
-                else if (KIND == KIND_II)
-                    ech[j] = expo[(int64_t)op.left * P.scale_stride + pj] + expo[(int64_t)op.right * P.scale_stride + pj];
+            const int64_t pj = (int64_t)tile * 64 + (pb0 + j) * 16 + l16;
+            if (KIND == KIND_TI) ech[j] = expo[(int64_t)op.right * P.scale_stride + pj];
+            else if (KIND == KIND_II)
+                ech[j] = expo[(int64_t)op.left * P.scale_stride + pj] + expo[(int64_t)op.right * P.scale_stride + pj];
+            else ech[j] = 0;
+        }
+        float bL[NB][KS], bR[NB][KS];
+        float tv[NB][XB][4];  // tip rows: left table (TI), left x right tables (TT)
+        auto load_block = [&](int j, int pb) {
+            const int col = pb * 16 + l16;
+#pragma unroll
+            for (int ks = 0; ks < KS; ks++) {
+                const int y = ks * 4 + lg, yr = y < N ? y : N - 1;
+                if (KIND != KIND_TT) {
+                    const float v = R[yr * 64 + col];
+                    bR[j][ks] = y < N ? v : 0.0f;
+                }
+                if (KIND == KIND_II) {
+                    const float v = L[yr * 64 + col];
+                    bL[j][ks] = y < N ? v : 0.0f;
+                }
             }
+            if (KIND != KIND_II) {
+                const float *tl = tabs + ((int64_t)stl[col] * ncat + c) * N;
+                const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)str[col] * ncat + c) * N;
+#pragma unroll
+                for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int x = xb * 16 + lg * 4 + r, xr = x < N ? x : N - 1;
+                        const float v = KIND == KIND_TT ? tl[xr] * tr[xr] : tl[xr];
+                        tv[j][xb][r] = (x < N) ? v : 0.0f;
+                    }
+            }
+        };
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int j = 0; j < NPB; j++) load_block(j, pb0 + j);
         }
 #pragma unroll
         for (int gi = 0; gi < NPB; gi += PBG) {
             const int g = pb0 + gi;
+            if constexpr (!HOIST) {
+#pragma unroll
+                for (int j = 0; j < PBG; j++) load_block(j, g + j);
+            }
             floatx4 out[PBG][XB];
+            const int jb = HOIST ? gi : 0;  // first held block of this group
 #pragma unroll
             for (int j = 0; j < PBG; j++) {
                 const int pb = g + j;
-                if (!active) {
-#pragma unroll
-                    for (int xb = 0; xb < XB; xb++) out[j][xb] = floatx4{0, 0, 0, 0};
-                } else if (KIND == KIND_TT) {
-                    const int sl = stl[pb * 16 + l16], sr = str[pb * 16 + l16];
-                    const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
-                    const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
+                if (KIND == KIND_TT) {
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++)
 #pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int x = xb * 16 + lg * 4 + r;
-                            out[j][xb][r] = (x < N) ? tl[x] * tr[x] : 0.0f;
-                        }
+                        for (int r = 0; r < 4; r++) out[j][xb][r] = tv[jb + j][xb][r];
                 } else {
                     // D_R = P_R . R (and D_L = P_L . L) on the matrix cores
                     floatx4 dL[XB], dR[XB];
@@ -168,31 +204,23 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                     }
 #pragma unroll
                     for (int ks = 0; ks < KS; ks++) {
-                        const int y = ks * 4 + lg;
-                        const float br = (y < N) ? R[y * 64 + pb * 16 + l16] : 0.0f;
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++)
-                            dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], br, dR[xb], 0, 0, 0);
+                            dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aR[xb][ks], bR[jb + j][ks], dR[xb], 0, 0, 0);
                         if (KIND == KIND_II) {
-                            const float bl = (y < N) ? L[y * 64 + pb * 16 + l16] : 0.0f;
 #pragma unroll
                             for (int xb = 0; xb < XB; xb++)
-                                dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[xb][ks], bl, dL[xb], 0, 0, 0);
+                                dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(aL[xb][ks], bL[jb + j][ks], dL[xb], 0, 0, 0);
                         }
                     }
                     if (KIND == KIND_II) {
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++) out[j][xb] = dL[xb] * dR[xb];
                     } else {
-                        const int sl = stl[pb * 16 + l16];
-                        const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++)
 #pragma unroll
-                            for (int r = 0; r < 4; r++) {
-                                const int x = xb * 16 + lg * 4 + r;
-                                out[j][xb][r] = (x < N) ? tl[x] * dR[xb][r] : 0.0f;
-                            }
+                            for (int r = 0; r < 4; r++) out[j][xb][r] = tv[jb + j][xb][r] * dR[xb][r];
                     }
                 }
                 // per-pattern maximum over this category (reduced over the 4 row groups)
@@ -203,7 +231,8 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                     for (int r = 0; r < 4; r++) m = fmaxf(m, fabsf(out[j][xb][r]));
                 m = fmaxf(m, __shfl_xor(m, 16, 64));
                 m = fmaxf(m, __shfl_xor(m, 32, 64));
-                if (lg == 0 && active) s_max[sp][c][pb * 16 + l16] = m;
+                // all four row groups store the same value; inactive waves use a spare row
+                s_max[sp][active ? c : PLK_GPU_MAX_CATS][pb * 16 + l16] = m;
             }
             __syncthreads();
             // maximum over all categories -> power-of-two renormalisation and exponent sum
@@ -316,10 +345,11 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
     __shared__ double s_wsum[16];
 
     for (int k = 0; k < P.nops; k++) {
-        const DevOp op = P.ops[k];
+        const int ko = op_index(P, tile, k);
+        const DevOp op = load_op(P.ops, ko);
         with_kind(op.kind, [&](auto KC) {
         constexpr int KIND = decltype(KC)::value;
-        const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
+        const float *__restrict__ M = (const float *)P.mats + (int64_t)ko * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         int sl = 0, sr = 0;
         if (KIND <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
@@ -486,8 +516,9 @@ __device__ inline void traverse_f32_valu_body(const KParams &P, const int tile) 
     constexpr int CPW = (PLK_GPU_MAX_CATS + 3) / 4;  // categories per wave at most
 
     for (int k = 0; k < P.nops; k++) {
-        const DevOp op = P.ops[k];
-        const float *__restrict__ M = (const float *)P.mats + (int64_t)k * P.mat_stride;
+        const int ko = op_index(P, tile, k);
+        const DevOp op = load_op(P.ops, ko);
+        const float *__restrict__ M = (const float *)P.mats + (int64_t)ko * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         int sl = 0, sr = 0;
         if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];

@@ -215,8 +215,9 @@ __device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
 
     // -------- post-order traversal (computePartialLikelihood for every op)
     for (int k = 0; k < P.nops; k++) {
-        const DevOp op = P.ops[k];
-        const double *__restrict__ M = (const double *)P.mats + (int64_t)k * P.mat_stride;
+        const int ko = op_index(P, tile, k);
+        const DevOp op = load_op(P.ops, ko);
+        const double *__restrict__ M = (const double *)P.mats + (int64_t)ko * P.mat_stride;
         double *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride;
         int sl = 0, sr = 0;
         if (op.kind <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];

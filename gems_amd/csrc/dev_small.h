@@ -170,14 +170,14 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
 
     int par = 0;
     for (int lv = 0; lv < P.nlevels; lv++) {
-        const int k0 = P.lvl[lv], k1 = P.lvl[lv + 1];
+        const int k0 = load_int(P.lvl, lv), k1 = load_int(P.lvl, lv + 1);
         for (int base = k0; base < k1; base += S, par ^= 1) {
             const int k = base + slot;
             const bool act = k < k1;
             DevOp op{};
             REAL out[CPW][4];
             if (act) {
-                op = P.ops[k];
+                op = load_op(P.ops, k);
                 const bool ltip = op.kind <= KIND_TI, rtip = op.kind == KIND_TT;
                 const int sl = ltip ? small_tip_state(P, L.state, op.left, op.lds_left, tile, lane) : 0;
                 const int sr = rtip ? small_tip_state(P, L.state, op.right, op.lds_right, tile, lane) : 0;

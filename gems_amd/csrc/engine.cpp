@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <queue>
 #include <vector>
 
 #include "engine_common.h"
@@ -267,6 +268,51 @@ static void plan_small(Ctx &c, bool branch, int br_a, int br_b, SmallPlan &sp) {
 // Upload the queued ops, run the P(t) kernel, then one traversal launch in `mode`.
 // The FP32 form also needs P(t) of the branch itself (an extra job after the ops) for
 // the logL stage, and computes theta in a separate FP64 kernel after the traversal.
+// A second valid order of an op list for half of the tiles (see KParams::perm): Kahn's
+// algorithm over the read-after-write dependencies, always taking the ready op with the
+// largest index (the mirror image of a post-order).  Only for lists in which no slot is
+// written twice or written after an earlier op read it; otherwise (and for short lists)
+// returns false and every tile keeps the list's order.  Each op computes the same values in
+// either order, so results do not depend on it.
+static bool alt_order(const Ctx &c, const std::vector<DevOp> &ops, std::vector<int32_t> &perm) {
+    const int n = (int)ops.size();
+    if (n < 8) return false;
+    std::vector<int> writer(c.nslots, -1), reader_last(c.nslots, -1);
+    std::vector<std::vector<int>> succ(n);
+    std::vector<int> indeg(n, 0);
+    auto reads = [&](const DevOp &o, int &a, int &b) {
+        a = o.kind == KIND_II ? o.left : -1;
+        b = o.kind != KIND_TT ? o.right : -1;
+    };
+    for (int j = 0; j < n; j++) {
+        int a, b;
+        reads(ops[j], a, b);
+        for (int s : {a, b}) {
+            if (s < 0) continue;
+            if (writer[s] >= 0) {
+                succ[writer[s]].push_back(j);
+                indeg[j]++;
+            }
+            reader_last[s] = j;
+        }
+        const int w = ops[j].out_slot;
+        if (writer[w] >= 0 || reader_last[w] >= 0) return false;  // WAW or WAR
+        writer[w] = j;
+    }
+    std::priority_queue<int> ready;
+    for (int j = 0; j < n; j++)
+        if (indeg[j] == 0) ready.push(j);
+    perm.clear();
+    while (!ready.empty()) {
+        const int j = ready.top();
+        ready.pop();
+        perm.push_back(j);
+        for (int k : succ[j])
+            if (--indeg[k] == 0) ready.push(k);
+    }
+    return (int)perm.size() == n;
+}
+
 static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     const int nops = (int)c.pending.size();
     const bool f32 = c.prec == PLK_FP32;
@@ -284,10 +330,13 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     SmallPlan sp;
     if (small) plan_small(c, mode != MODE_PARTIALS, p.br_a, p.br_b, sp);
     if (small) c.jobs.clear();  // the level-parallel kernel forms P(t) itself
+    std::vector<int32_t> perm;
+    const bool two_orders = !small && alt_order(c, c.pending, perm);
+    p.perm = nullptr;
     const int njobs = (int)c.jobs.size();
-    if (njobs > 0 || small) {
+    if (njobs > 0 || small || two_orders) {
         size_t ob = sizeof(DevOp) * nops, jb = sizeof(PJob) * njobs;
-        size_t xb = small ? sizeof(int32_t) * (sp.lvl.size() + sp.tips.size()) : 0;
+        size_t xb = small ? sizeof(int32_t) * (sp.lvl.size() + sp.tips.size()) : two_orders ? sizeof(int32_t) * nops : 0;
         rc = alloc_stage(c, ob + xb + jb);
         if (rc) return rc;
         char *h = (char *)c.h_stage;
@@ -302,6 +351,10 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
             p.nbuf = sp.nbuf;
             p.br_lds_a = sp.br_lds_a;
             p.br_lds_b = sp.br_lds_b;
+        }
+        if (two_orders) {
+            std::memcpy(h + ob, perm.data(), sizeof(int32_t) * nops);
+            p.perm = (const int32_t *)((char *)c.d_ops + ob);
         }
         if (njobs > 0) std::memcpy(h + ob + xb, c.jobs.data(), jb);
         if (ob + xb > 0) HIP_TRY(hipMemcpyAsync(c.d_ops, h, ob + xb, hipMemcpyHostToDevice, c.stream));

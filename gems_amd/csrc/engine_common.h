@@ -72,6 +72,10 @@ struct KParams {
     double *result;           // 2
     // level-parallel 4-state kernel (ops sorted by dependency level)
     const int32_t *lvl;       // nlevels + 1 op-index boundaries
+    // a second valid order of the ops (nops indices) that half of the tiles follow, or
+    // null: tiles walking the same op sequence in step would alternate chip-wide between
+    // a memory burst and a compute phase; two orders interleave the two
+    const int32_t *perm;
     int nlevels;
     int nbuf;                 // LDS partial buffers
     int ntip;                 // tips whose states are staged in LDS

@@ -10,10 +10,15 @@ __global__ void __launch_bounds__(256) k_pmatrix_f32(KParams P, const PJob *__re
     pmatrix_f32_body<N>(P, jobs[blockIdx.x]);
 }
 
-// WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1 (each wave
-// may use the whole 512-entry register file); WIDE allows 5-8 categories or WPC > 1.
+// WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1; WIDE
+// allows 5-8 categories or WPC > 1.  The 20-state forms keep four waves per SIMD resident
+// (128 registers): at three, a 49k-pattern alignment's 770 tiles would overflow the 768
+// workgroup slots of one round; the 61-state form may use the whole register file.
+template <int N, int WIDE>
+constexpr int mfma_min_waves() { return (N <= 32 && !WIDE) ? 4 : 1; }
+
 template <int N, int MODE, int WIDE, int WPC>
-__global__ void __launch_bounds__(WIDE ? 64 * 4 * 4 : 256) k_traverse_f32_mfma(KParams P) {
+__global__ void __launch_bounds__(WIDE ? 64 * 4 * 4 : 256, (mfma_min_waves<N, WIDE>())) k_traverse_f32_mfma(KParams P) {
     traverse_f32_mfma_body<N, MODE, WPC>(P, blockIdx.x);
 }
 

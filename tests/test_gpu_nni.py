@@ -53,10 +53,11 @@ def test_newton_multi_matches_single(prec, safe):
         lt = ctypes.c_double()
         assert G.lib().plk_loglik_from_theta(ctx, out[i], ctypes.byref(lt)) == 0
         assert rel(logl[i], lt.value) <= 1e-12
-    # eval-only mode returns the logL at the guesses
+    # eval-only mode returns the logL at the guesses (from theta in FP64; the FP32 form's
+    # branch kernel sums in FP32, so it agrees to the FP32 form's 1e-6 bar)
     _, l0, _ = g.newton_multi(refs, guesses, max_nr=-1)
     lg = g.loglik(*br[0])
-    assert rel(l0[0], lg) <= 1e-12
+    assert rel(l0[0], lg) <= (1e-12 if prec == G.FP64 else 1e-6)
 
 
 @pytest.mark.parametrize("prec", [G.FP64, G.FP32], ids=["f64", "f32"])
