@@ -68,6 +68,25 @@ __device__ __forceinline__ int op_index(const KParams &P, int tile, int k) {
     return load_int(P.perm, k);
 }
 
+// Tip states of one tile staged in LDS when a traversal starts (the first kStageTaxa taxa;
+// later ones are read from global memory): a tip child's state then costs an LDS read
+// instead of a memory round trip that its leaf-table gather would wait behind.  The
+// barriers on both sides let persistent (batched) launches restage for every tile.
+constexpr int kStageTaxa = 128;
+__device__ __forceinline__ void stage_tips(const KParams &P, int tile, uint32_t *s) {
+    const int nst = P.ntaxa < kStageTaxa ? P.ntaxa : kStageTaxa;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nst * 16; i += blockDim.x) {
+        const int t = i >> 4, q = i & 15;
+        s[i] = ((const uint32_t *)(P.states + (int64_t)t * P.nptn_pad + (int64_t)tile * 64))[q];
+    }
+    __syncthreads();
+}
+__device__ __forceinline__ int tip_state(const KParams &P, const uint32_t *s, int taxon, int tile, int col) {
+    return taxon < kStageTaxa ? ((const uint8_t *)s)[taxon * 64 + col]
+                              : P.states[(int64_t)taxon * P.nptn_pad + (int64_t)tile * 64 + col];
+}
+
 __device__ __forceinline__ int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); }
 __device__ __forceinline__ int nwaves() { return blockDim.x >> 6; }
 

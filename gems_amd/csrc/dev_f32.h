@@ -21,14 +21,15 @@ namespace dev {
 // ---------------------------------------------------------------------------------------
 // P(t) in FP32 form: P_c[x][y] (FP64 math) written as MFMA A fragments
 // frag[c][xb][ks][lane] = P_c[xb*16 + lane%16][ks*4 + lane/16] (zero padded), or plainly
-// [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][x] = (P_c tip_s)[x].
+// [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][x] = (P_c tip_s)[x], rows
+// f32_trow(N) long (zero padded for the matrix-core forms).
 template <int N>
 __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
-    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
     const int ncat = P.ncat, su = P.su;
     float *M = (float *)P.mats + (int64_t)j.op * P.mat_stride;
     float *F = M + (int64_t)j.side * ncat * FR;
-    float *T = M + (int64_t)2 * ncat * FR + (int64_t)j.side * (su + 1) * ncat * N;
+    float *T = M + (int64_t)2 * ncat * FR + (int64_t)j.side * (su + 1) * ncat * TROW;
     __shared__ double ex[N];
     __shared__ double Pm[N * N];
     for (int c = 0; c < ncat; c++) {
@@ -51,13 +52,14 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
             }
         }
         if (j.leaf) {
-            for (int idx = threadIdx.x; idx < su * N; idx += blockDim.x) {
-                const int s = idx / N, x = idx % N;
+            for (int idx = threadIdx.x; idx < su * TROW; idx += blockDim.x) {
+                const int s = idx / TROW, x = idx % TROW;
                 double v = 0.0;
-                for (int y = 0; y < N; y++) v = fma(Pm[x * N + y], P.tipvec[s * N + y], v);
-                T[((int64_t)s * ncat + c) * N + x] = (float)v;
+                if (x < N)
+                    for (int y = 0; y < N; y++) v = fma(Pm[x * N + y], P.tipvec[s * N + y], v);
+                T[((int64_t)s * ncat + c) * TROW + x] = (float)v;
             }
-            for (int x = threadIdx.x; x < N; x += blockDim.x) T[((int64_t)su * ncat + c) * N + x] = 1.0f;
+            for (int x = threadIdx.x; x < TROW; x += blockDim.x) T[((int64_t)su * ncat + c) * TROW + x] = x < N ? 1.0f : 0.0f;
         }
         __syncthreads();
     }
@@ -74,7 +76,7 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
 // pattern blocks [(w % WPC) * 4/WPC, ...).
 template <int N, int MODE, int WPC = 1>
 __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) {
-    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
     constexpr int NPB = 4 / WPC;  // pattern blocks per wave
     const int lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
     const int ncat = P.ncat;
@@ -90,7 +92,9 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     __shared__ float s_max[2][PLK_GPU_MAX_CATS + 1][64];
     __shared__ double s_cat[PLK_GPU_MAX_CATS][64];
     __shared__ float s_pi[64];
+    __shared__ uint32_t s_tip[kStageTaxa * 16];
     if (MODE == MODE_LOGL && threadIdx.x < N) s_pi[threadIdx.x] = (float)P.pi[threadIdx.x];
+    stage_tips(P, tile, s_tip);
 
     // pattern blocks per renormalisation group (outputs held in registers across the
     // category exchange): two for <= 32 states, whose loads for all of the wave's blocks
@@ -109,8 +113,6 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
         constexpr int KIND = decltype(KC)::value;
         const float *__restrict__ M = (const float *)P.mats + (int64_t)ko * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
-        const uint8_t *stl = P.states + (int64_t)op.left * P.nptn_pad + (int64_t)tile * 64;
-        const uint8_t *str = P.states + (int64_t)op.right * P.nptn_pad + (int64_t)tile * 64;
         const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
         const float *__restrict__ L = part + (int64_t)op.left * P.slot_stride + cbase;
         // Every global load of an op (A fragments, the children's columns, tip states and
@@ -145,7 +147,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             else ech[j] = 0;
         }
         float bL[NB][KS], bR[NB][KS];
-        float tv[NB][XB][4];  // tip rows: left table (TI), left x right tables (TT)
+        floatx4 tv[NB][XB];  // tip rows: left table (TI), left x right tables (TT)
         auto load_block = [&](int j, int pb) {
             const int col = pb * 16 + l16;
 #pragma unroll
@@ -161,16 +163,13 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 }
             }
             if (KIND != KIND_II) {
-                const float *tl = tabs + ((int64_t)stl[col] * ncat + c) * N;
-                const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)str[col] * ncat + c) * N;
+                const int sl = tip_state(P, s_tip, op.left, tile, col);
+                const int sr = KIND == KIND_TT ? tip_state(P, s_tip, op.right, tile, col) : 0;
+                const floatx4 *tl = (const floatx4 *)(tabs + ((int64_t)sl * ncat + c) * TROW + lg * 4);
+                const floatx4 *tr =
+                    (const floatx4 *)(tabs + (int64_t)(P.su + 1) * ncat * TROW + ((int64_t)sr * ncat + c) * TROW + lg * 4);
 #pragma unroll
-                for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int x = xb * 16 + lg * 4 + r, xr = x < N ? x : N - 1;
-                        const float v = KIND == KIND_TT ? tl[xr] * tr[xr] : tl[xr];
-                        tv[j][xb][r] = (x < N) ? v : 0.0f;
-                    }
+                for (int xb = 0; xb < XB; xb++) tv[j][xb] = KIND == KIND_TT ? tl[xb * 4] * tr[xb * 4] : tl[xb * 4];
             }
         };
         if constexpr (HOIST) {
@@ -191,9 +190,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 const int pb = g + j;
                 if (KIND == KIND_TT) {
 #pragma unroll
-                    for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                        for (int r = 0; r < 4; r++) out[j][xb][r] = tv[jb + j][xb][r];
+                    for (int xb = 0; xb < XB; xb++) out[j][xb] = tv[jb + j][xb];
                 } else {
                     // D_R = P_R . R (and D_L = P_L . L) on the matrix cores
                     floatx4 dL[XB], dR[XB];
@@ -218,9 +215,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                         for (int xb = 0; xb < XB; xb++) out[j][xb] = dL[xb] * dR[xb];
                     } else {
 #pragma unroll
-                        for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                            for (int r = 0; r < 4; r++) out[j][xb][r] = tv[jb + j][xb][r] * dR[xb][r];
+                        for (int xb = 0; xb < XB; xb++) out[j][xb] = tv[jb + j][xb] * dR[xb];
                     }
                 }
                 // per-pattern maximum over this category (reduced over the 4 row groups)
@@ -330,7 +325,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 // above them.
 template <int N, int MODE>
 __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
-    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64;
+    constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
     constexpr int NC = 4;
     const int lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
     const int ncat = P.ncat;
@@ -343,6 +338,8 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
     const int64_t p = (int64_t)tile * 64 + pl;
     auto cbase = [&](int c) { return ((int64_t)tile * ncat + c) * N * 64 + pbc * 16 + l16; };
     __shared__ double s_wsum[16];
+    __shared__ uint32_t s_tip[kStageTaxa * 16];
+    stage_tips(P, tile, s_tip);
 
     for (int k = 0; k < P.nops; k++) {
         const int ko = op_index(P, tile, k);
@@ -352,8 +349,8 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
         const float *__restrict__ M = (const float *)P.mats + (int64_t)ko * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         int sl = 0, sr = 0;
-        if (KIND <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
-        if (KIND == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
+        if (KIND <= KIND_TI) sl = tip_state(P, s_tip, op.left, tile, pl);
+        if (KIND == KIND_TT) sr = tip_state(P, s_tip, op.right, tile, pl);
         floatx4 out[NC][XB];
         float m = 0.0f;
 #pragma unroll
@@ -362,15 +359,11 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
             for (int xb = 0; xb < XB; xb++) out[c][xb] = floatx4{0, 0, 0, 0};
             if (c >= ncat || !active) continue;
             if (KIND == KIND_TT) {
-                const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
-                const float *tr = tabs + (int64_t)(P.su + 1) * ncat * N + ((int64_t)sr * ncat + c) * N;
+                const floatx4 *tl = (const floatx4 *)(tabs + ((int64_t)sl * ncat + c) * TROW + lg * 4);
+                const floatx4 *tr =
+                    (const floatx4 *)(tabs + (int64_t)(P.su + 1) * ncat * TROW + ((int64_t)sr * ncat + c) * TROW + lg * 4);
 #pragma unroll
-                for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int x = xb * 16 + lg * 4 + r;
-                        out[c][xb][r] = (x < N) ? tl[x] * tr[x] : 0.0f;
-                    }
+                for (int xb = 0; xb < XB; xb++) out[c][xb] = tl[xb * 4] * tr[xb * 4];
             } else {
                 const float *__restrict__ FRm = M + (int64_t)(ncat + c) * FR;
                 const float *__restrict__ FLm = M + (int64_t)c * FR;
@@ -400,14 +393,9 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++) out[c][xb] = dL[xb] * dR[xb];
                 } else {
-                    const float *tl = tabs + ((int64_t)sl * ncat + c) * N;
+                    const floatx4 *tl = (const floatx4 *)(tabs + ((int64_t)sl * ncat + c) * TROW + lg * 4);
 #pragma unroll
-                    for (int xb = 0; xb < XB; xb++)
-#pragma unroll
-                        for (int r = 0; r < 4; r++) {
-                            const int x = xb * 16 + lg * 4 + r;
-                            out[c][xb][r] = (x < N) ? tl[x] * dR[xb][r] : 0.0f;
-                        }
+                    for (int xb = 0; xb < XB; xb++) out[c][xb] = tl[xb * 4] * dR[xb];
                 }
             }
 #pragma unroll

@@ -40,7 +40,7 @@ static bool supported_states(int n) { return n == 4 || n == 20 || n == 61; }
 static int64_t mat_stride_of(int n, int ncat, int su, int prec) {
     if (prec == PLK_FP32) {
         int64_t fr = (int64_t)f32_xblocks(n) * f32_ksteps(n) * 64;
-        return round_up(2LL * ncat * fr + 2LL * (su + 1) * n * ncat, 64);
+        return round_up(2LL * ncat * fr + 2LL * (su + 1) * f32_trow(n) * ncat, 64);
     }
     return round_up(2LL * ncat * n * n + 2LL * (su + 1) * n * ncat, 32);
 }
@@ -93,6 +93,7 @@ KParams make_params(Ctx &c) {
     p.scale = c.d_scale;
     p.scale_stride = (int64_t)c.nptn_pad * ((c.safe && c.prec == PLK_FP64) ? c.ncat : 1);
     p.states = c.d_states;
+    p.ntaxa = c.ntaxa;
     p.freq = c.d_freq;
     p.invar = c.d_invar;
     p.mats = c.d_mats;

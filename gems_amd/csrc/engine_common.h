@@ -15,6 +15,9 @@
 // FP32-form matrix-core tiling: 16-row x blocks and 4-deep k steps of a state count.
 constexpr int f32_xblocks(int n) { return (n + 15) / 16; }
 constexpr int f32_ksteps(int n) { return (n + 3) / 4; }
+// FP32 leaf-table row length: the matrix-core forms pad a row to whole 16-row blocks
+// (zeros past n), so a lane's four output rows are one aligned 16-byte load
+constexpr int f32_trow(int n) { return n == 4 ? 4 : f32_xblocks(n) * 16; }
 
 namespace plk {
 
@@ -55,6 +58,7 @@ struct KParams {
     void *scale;              // slots x scale_stride counters (uint16 FP64 / int32 FP32)
     int64_t scale_stride;
     const uint8_t *states;    // ntaxa x nptn_pad
+    int ntaxa;
     const double *freq, *invar;
     const void *mats;         // per-op matrices
     int64_t mat_stride;       // reals per op
