@@ -20,7 +20,7 @@ namespace dev {
 
 // ---------------------------------------------------------------------------------------
 // P(t) in FP32 form: P_c[x][y] (FP64 math) written as MFMA A fragments
-// frag[c][xb][ks][lane] = P_c[xb*16 + lane%16][ks*4 + lane/16] (zero padded), or plainly
+// frag[c][xb][ks][lane] = P_c[xb*16 + lane%16][f32_kstate(ks, lane/16)] (zero padded), or plainly
 // [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][x] = (P_c tip_s)[x], rows
 // f32_trow(N) long (zero padded for the matrix-core forms).
 template <int N>
@@ -47,7 +47,7 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
         } else {
             for (int idx = threadIdx.x; idx < FR; idx += blockDim.x) {
                 const int xb = idx / (KS * 64), ks = (idx / 64) % KS, l = idx % 64;
-                const int x = xb * 16 + (l & 15), y = ks * 4 + (l >> 4);
+                const int x = xb * 16 + (l & 15), y = f32_kstate(N, ks, l >> 4);
                 F[(int64_t)c * FR + idx] = (x < N && y < N) ? (float)Pm[x * N + y] : 0.0f;
             }
         }
@@ -150,16 +150,32 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
         floatx4 tv[NB][XB];  // tip rows: left table (TI), left x right tables (TT)
         auto load_block = [&](int j, int pb) {
             const int col = pb * 16 + l16;
-#pragma unroll
-            for (int ks = 0; ks < KS; ks++) {
-                const int y = ks * 4 + lg, yr = y < N ? y : N - 1;
+            if constexpr (f32_rg(N)) {
+                // row group lg (k-steps 0-3) as one 16-byte load, state 16 + lg (k-step 4)
                 if (KIND != KIND_TT) {
-                    const float v = R[yr * 64 + col];
-                    bR[j][ks] = y < N ? v : 0.0f;
+                    const floatx4 q = *(const floatx4 *)(R + (lg * 64 + col) * 4);
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++) bR[j][ks] = q[ks];
+                    bR[j][4] = R[(4 * 64 + col) * 4 + lg];
                 }
                 if (KIND == KIND_II) {
-                    const float v = L[yr * 64 + col];
-                    bL[j][ks] = y < N ? v : 0.0f;
+                    const floatx4 q = *(const floatx4 *)(L + (lg * 64 + col) * 4);
+#pragma unroll
+                    for (int ks = 0; ks < 4; ks++) bL[j][ks] = q[ks];
+                    bL[j][4] = L[(4 * 64 + col) * 4 + lg];
+                }
+            } else {
+#pragma unroll
+                for (int ks = 0; ks < KS; ks++) {
+                    const int y = ks * 4 + lg, yr = y < N ? y : N - 1;
+                    if (KIND != KIND_TT) {
+                        const float v = R[yr * 64 + col];
+                        bR[j][ks] = y < N ? v : 0.0f;
+                    }
+                    if (KIND == KIND_II) {
+                        const float v = L[yr * 64 + col];
+                        bL[j][ks] = y < N ? v : 0.0f;
+                    }
                 }
             }
             if (KIND != KIND_II) {
@@ -237,13 +253,25 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 float pm = 0.0f;
                 for (int cc = 0; cc < ncat; cc++) pm = fmaxf(pm, s_max[sp][cc][pl]);
                 const int e = norm_exponent(pm);
-#pragma unroll
-                for (int xb = 0; xb < XB; xb++)
+                if constexpr (f32_rg(N)) {
+                    // rows 4lg..4lg+3 (row group lg) and, in lane group 0, rows 16-19
+                    floatx4 v0, v1;
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int x = xb * 16 + lg * 4 + r;
-                        if (x < N && active) outp[x * 64 + pl] = ldexpf(out[j][xb][r], -e);
+                        v0[r] = ldexpf(out[j][0][r], -e);
+                        v1[r] = ldexpf(out[j][1][r], -e);
                     }
+                    if (active) *(floatx4 *)(outp + (lg * 64 + pl) * 4) = v0;
+                    if (active && lg == 0) *(floatx4 *)(outp + (4 * 64 + pl) * 4) = v1;
+                } else {
+#pragma unroll
+                    for (int xb = 0; xb < XB; xb++)
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int x = xb * 16 + lg * 4 + r;
+                            if (x < N && active) outp[x * 64 + pl] = ldexpf(out[j][xb][r], -e);
+                        }
+                }
                 if (cw == 0 && lg == 0)  // one wave per pattern block keeps the exponents
                     expo[(int64_t)op.out_slot * P.scale_stride + (int64_t)tile * 64 + pl] = e + ech[gi + j];
             }
@@ -279,8 +307,8 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
         for (int xb = 0; xb < XB; xb++) d[xb] = floatx4{0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const int y = ks * 4 + lg;
-            const float b = (y < N) ? B[y * 64 + pb * 16 + l16] : 0.0f;
+            const int y = f32_kstate(N, ks, lg);
+            const float b = (y < N) ? B[f32_xoff(N, y) + f32_loff(N, pb * 16 + l16)] : 0.0f;
 #pragma unroll
             for (int xb = 0; xb < XB; xb++) d[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(at[xb][ks], b, d[xb], 0, 0, 0);
         }
@@ -292,7 +320,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             for (int r = 0; r < 4; r++) {
                 const int x = xb * 16 + lg * 4 + r;
                 if (x < N) {
-                    const float a = atip ? (float)tv[x] : A[x * 64 + pb * 16 + l16];
+                    const float a = atip ? (float)tv[x] : A[f32_xoff(N, x) + f32_loff(N, pb * 16 + l16)];
                     acc = fmaf(s_pi[x] * a, d[xb][r], acc);
                 }
             }
@@ -336,7 +364,7 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
     int *__restrict__ expo = (int *)P.scale;
     const int pl = pbc * 16 + l16;                       // pattern within the tile
     const int64_t p = (int64_t)tile * 64 + pl;
-    auto cbase = [&](int c) { return ((int64_t)tile * ncat + c) * N * 64 + pbc * 16 + l16; };
+    auto cbase = [&](int c) { return ((int64_t)tile * ncat + c) * N * 64 + f32_loff(N, pbc * 16 + l16); };
     __shared__ double s_wsum[16];
     __shared__ uint32_t s_tip[kStageTaxa * 16];
     stage_tips(P, tile, s_tip);
@@ -377,13 +405,13 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
                 }
 #pragma unroll
                 for (int ks = 0; ks < KS; ks++) {
-                    const int y = ks * 4 + lg;
-                    const float br = (y < N) ? R[y * 64] : 0.0f;
+                    const int y = f32_kstate(N, ks, lg);
+                    const float br = (y < N) ? R[f32_xoff(N, y)] : 0.0f;
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++)
                         dR[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(FRm[(xb * KS + ks) * 64 + lane], br, dR[xb], 0, 0, 0);
                     if (KIND == KIND_II) {
-                        const float bl = (y < N) ? L[y * 64] : 0.0f;
+                        const float bl = (y < N) ? L[f32_xoff(N, y)] : 0.0f;
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++)
                             dL[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(FLm[(xb * KS + ks) * 64 + lane], bl, dL[xb], 0, 0, 0);
@@ -416,7 +444,7 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         const int x = xb * 16 + lg * 4 + r;
-                        if (x < N) outp[cbase(c) + x * 64] = ldexpf(out[c][xb][r], -e);
+                        if (x < N) outp[cbase(c) + f32_xoff(N, x)] = ldexpf(out[c][xb][r], -e);
                     }
             }
             if (lg == 0) {
@@ -449,8 +477,8 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
         for (int xb = 0; xb < XB; xb++) d[xb] = floatx4{0, 0, 0, 0};
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
-            const int y = ks * 4 + lg;
-            const float b = (y < N) ? B[y * 64] : 0.0f;
+            const int y = f32_kstate(N, ks, lg);
+            const float b = (y < N) ? B[f32_xoff(N, y)] : 0.0f;
 #pragma unroll
             for (int xb = 0; xb < XB; xb++) d[xb] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ft[(xb * KS + ks) * 64 + lane], b, d[xb], 0, 0, 0);
         }
@@ -461,7 +489,7 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
             for (int r = 0; r < 4; r++) {
                 const int x = xb * 16 + lg * 4 + r;
                 if (x < N) {
-                    const float a = atip ? (float)P.tipvec[sa * N + x] : A[x * 64];
+                    const float a = atip ? (float)P.tipvec[sa * N + x] : A[f32_xoff(N, x)];
                     acc = fmaf((float)P.pi[x] * a, d[xb][r], acc);
                 }
             }
@@ -645,14 +673,15 @@ __device__ inline bool theta_f32_body(const KParams &P, const int tile) {
     }
     __syncthreads();
     for (int c = w; c < ncat; c += ncw) {
-        const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+        const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;  // theta (FP64 layout)
+        const int64_t offf = ((int64_t)tile * ncat + c) * N * 64 + f32_loff(N, lane);
         double a[N], b[N];
-        const float *__restrict__ B = partf + (int64_t)P.br_b * P.slot_stride + off;
-        const float *__restrict__ A = partf + (int64_t)(atip ? 0 : P.br_a) * P.slot_stride + off;
+        const float *__restrict__ B = partf + (int64_t)P.br_b * P.slot_stride + offf;
+        const float *__restrict__ A = partf + (int64_t)(atip ? 0 : P.br_a) * P.slot_stride + offf;
 #pragma unroll
         for (int x = 0; x < N; x++) {
-            b[x] = (double)B[x * 64];
-            a[x] = atip ? P.tipvec[sa * N + x] : (double)A[x * 64];
+            b[x] = (double)B[f32_xoff(N, x)];
+            a[x] = atip ? P.tipvec[sa * N + x] : (double)A[f32_xoff(N, x)];
         }
         double l0 = 0.0, l1 = 0.0, l2 = 0.0;
         for_states<N>([&](const int i) {

@@ -1005,9 +1005,14 @@ extern "C" int plk_get_partial(plk_ctx *c, int slot, double *partial, uint16_t *
         } else {
             HIP_TRY(hipMemcpy(raw.data(), (char *)c->d_part + 8 * slot_len * slot, 8 * slot_len, hipMemcpyDeviceToHost));
         }
+        const bool f32 = c->prec == PLK_FP32;
         for (int p = 0; p < c->nptn; p++)
-            for (int i = 0; i < c->block; i++)
-                partial[(int64_t)p * c->block + i] = raw[((int64_t)(p / 64) * c->block + i) * 64 + p % 64];
+            for (int i = 0; i < c->block; i++) {
+                const int cc = i / c->n, x = i % c->n;
+                const int64_t base = ((int64_t)(p / 64) * c->block + (int64_t)cc * c->n) * 64;
+                partial[(int64_t)p * c->block + i] =
+                    f32 ? raw[base + f32_xoff(c->n, x) + f32_loff(c->n, p % 64)] : raw[base + (int64_t)x * 64 + p % 64];
+            }
     }
     if (scale && c->prec == PLK_FP32) return fail(PLK_ERR_ARG, "plk_get_partial: the FP32 form keeps exponents (plk_get_exponents)");
     if (scale) {

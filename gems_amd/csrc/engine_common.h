@@ -18,6 +18,18 @@ constexpr int f32_ksteps(int n) { return (n + 3) / 4; }
 // FP32 leaf-table row length: the matrix-core forms pad a row to whole 16-row blocks
 // (zeros past n), so a lane's four output rows are one aligned 16-byte load
 constexpr int f32_trow(int n) { return n == 4 ? 4 : f32_xblocks(n) * 16; }
+// FP32 partial layout inside one (tile, category) block of 64 patterns.  20 states:
+// row groups of 4 states, element (x, p) at ((x / 4) * 64 + p) * 4 + x % 4, so the four
+// matrix-core output rows a lane holds (and the four B-operand rows it feeds) are one
+// 16-byte access; other state counts: element (x, p) at x * 64 + p.
+constexpr bool f32_rg(int n) { return n == 20; }
+__host__ __device__ constexpr int f32_xoff(int n, int x) { return f32_rg(n) ? (x >> 2) * 256 + (x & 3) : x * 64; }
+__host__ __device__ constexpr int f32_loff(int n, int p) { return f32_rg(n) ? p * 4 : p; }
+// A-fragment k-order of the 20-state form: k-step ks < 4 of lane group g reads state
+// 4g + ks (row group g, element ks), k-step 4 reads state 16 + g (row group 4)
+__host__ __device__ constexpr int f32_kstate(int n, int ks, int g) {
+    return f32_rg(n) ? (ks < 4 ? 4 * g + ks : 16 + g) : ks * 4 + g;
+}
 
 namespace plk {
 
