@@ -34,6 +34,11 @@ struct SmallLds {
     uint8_t *state;  // ntip x 64
     double *tipv;    // (su + 1) x 4 tip vectors (FP64 form: eigen-space tip_lh, FP32 form: real)
     double *val;     // ncat x 4 branch vectors (FP64 logL stage), padded to 3 x
+    // staged op list (P.small_stage): nops x 12 words, nlevels + 1 level bounds, and
+    // ex[op][side][c][i] = exp(lambda_i r_c t_side)
+    int32_t *ops_w;
+    int32_t *lvl;
+    double *ex;
 };
 
 // Carve the dynamic LDS block; the host sizes it with the same formula (small_lds_bytes).
@@ -52,6 +57,14 @@ __device__ __forceinline__ SmallLds<REAL> small_lds(const KParams &P, char *base
     s.scale = (int *)(base + off);
     off += (size_t)P.nbuf * scw * 64 * sizeof(int);
     s.state = (uint8_t *)(base + off);
+    off += (size_t)P.ntip * 64;
+    off = (off + 15) & ~(size_t)15;
+    s.ops_w = (int32_t *)(base + off);
+    off += (size_t)P.nops * sizeof(DevOp);
+    s.lvl = (int32_t *)(base + off);
+    off += (size_t)(P.nlevels + 1) * 4;
+    off = (off + 15) & ~(size_t)15;
+    s.ex = (double *)(base + off);
     return s;
 }
 
@@ -87,6 +100,37 @@ __device__ __forceinline__ void small_exp(const KParams &P, int c, double t, dou
     const cdptr ev = as_const(P.eval), rt = as_const(P.rates);
 #pragma unroll
     for (int i = 0; i < 4; i++) ex[i] = exp(ev[i] * (rt[c] * t));
+}
+
+// The exponentials of op k's child `side` in category c: from the staged table, else
+// computed here (the same expression either way).
+template <typename REAL>
+__device__ __forceinline__ void small_exp_op(const KParams &P, const SmallLds<REAL> &L, int k, int side, int c, double t,
+                                             double (&ex)[4]) {
+    if (P.small_stage) {
+        const double *e = L.ex + (((size_t)k * 2 + side) * P.ncat + c) * 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++) ex[i] = e[i];
+    } else {
+        small_exp(P, c, t, ex);
+    }
+}
+
+template <typename REAL>
+__device__ __forceinline__ DevOp small_op(const KParams &P, const SmallLds<REAL> &L, int k) {
+    if (!P.small_stage) return load_op(P.ops, k);
+    const int32_t *w = L.ops_w + (size_t)k * (sizeof(DevOp) / 4);
+    DevOp o;
+    o.out_slot = __builtin_amdgcn_readfirstlane(w[0]);
+    o.kind = __builtin_amdgcn_readfirstlane(w[1]);
+    o.left = __builtin_amdgcn_readfirstlane(w[2]);
+    o.right = __builtin_amdgcn_readfirstlane(w[3]);
+    o.lds_out = __builtin_amdgcn_readfirstlane(w[4]);
+    o.lds_left = __builtin_amdgcn_readfirstlane(w[5]);
+    o.lds_right = __builtin_amdgcn_readfirstlane(w[6]);
+    o.level = __builtin_amdgcn_readfirstlane(w[7]);
+    o.len_left = o.len_right = 0.0;  // the exponentials are staged
+    return o;
 }
 
 __device__ __forceinline__ void small_pmat(const KParams &P, const double (&ex)[4], double (&Pd)[16]) {
@@ -158,6 +202,17 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
         L.tipv[i] = FORM == 0 ? P.tip_lh[i] : P.tipvec[i];
     for (int i = threadIdx.x; i < P.ntip * 64; i += blockDim.x)
         L.state[i] = P.states[(int64_t)P.tips[i >> 6] * P.nptn_pad + (int64_t)tile * 64 + (i & 63)];
+    if (P.small_stage) {
+        const int nw = P.nops * (int)(sizeof(DevOp) / 4);
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) L.ops_w[i] = ((const int32_t *)P.ops)[i];
+        for (int i = threadIdx.x; i <= P.nlevels; i += blockDim.x) L.lvl[i] = P.lvl[i];
+        const int ne = P.nops * 2 * ncat * 4;
+        for (int idx = threadIdx.x; idx < ne; idx += blockDim.x) {
+            const int i = idx & 3, c = (idx >> 2) % ncat, side = (idx / (4 * ncat)) & 1, k = idx / (8 * ncat);
+            const double t = side ? P.ops[k].len_right : P.ops[k].len_left;
+            L.ex[idx] = exp(P.eval[i] * (P.rates[c] * t));
+        }
+    }
     if (FORM == 0 && MODE == MODE_LOGL) {
         // val[c][i] = exp(lambda_i r_c t) w_c (tree/phylokernelnew.h:2731-2766)
         for (int idx = threadIdx.x; idx < ncat * 4; idx += blockDim.x) {
@@ -170,14 +225,15 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
 
     int par = 0;
     for (int lv = 0; lv < P.nlevels; lv++) {
-        const int k0 = load_int(P.lvl, lv), k1 = load_int(P.lvl, lv + 1);
+        const int k0 = P.small_stage ? __builtin_amdgcn_readfirstlane(L.lvl[lv]) : load_int(P.lvl, lv);
+        const int k1 = P.small_stage ? __builtin_amdgcn_readfirstlane(L.lvl[lv + 1]) : load_int(P.lvl, lv + 1);
         for (int base = k0; base < k1; base += S, par ^= 1) {
             const int k = base + slot;
             const bool act = k < k1;
             DevOp op{};
             REAL out[CPW][4];
             if (act) {
-                op = load_op(P.ops, k);
+                op = small_op(P, L, k);
                 const bool ltip = op.kind <= KIND_TI, rtip = op.kind == KIND_TT;
                 const int sl = ltip ? small_tip_state(P, L.state, op.left, op.lds_left, tile, lane) : 0;
                 const int sr = rtip ? small_tip_state(P, L.state, op.right, op.lds_right, tile, lane) : 0;
@@ -192,8 +248,8 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
                     if (!rtip) small_child_vals<REAL>(P, L, op.right, op.lds_right, c, tile, lane, rv4);
                     if constexpr (FORM == 0) {
                         double exl[4], exr[4];
-                        small_exp(P, c, op.len_left, exl);
-                        small_exp(P, c, op.len_right, exr);
+                        small_exp_op(P, L, k, 0, c, op.len_left, exl);
+                        small_exp_op(P, L, k, 1, c, op.len_right, exr);
                         const cdptr uinv = as_const(P.inv_evec);
                         double t[4];
 #pragma unroll
@@ -229,9 +285,9 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
                         }
                     } else {
                         double ex[4], PL[16], PR[16];
-                        small_exp(P, c, op.len_left, ex);
+                        small_exp_op(P, L, k, 0, c, op.len_left, ex);
                         small_pmat(P, ex, PL);
-                        small_exp(P, c, op.len_right, ex);
+                        small_exp_op(P, L, k, 1, c, op.len_right, ex);
                         small_pmat(P, ex, PR);
 #pragma unroll
                         for (int x = 0; x < 4; x++) {

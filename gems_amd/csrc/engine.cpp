@@ -145,6 +145,7 @@ static void timing_collect(Ctx &c, const bool used[3]) {
 struct SmallPlan {
     std::vector<int32_t> lvl, tips;
     int nbuf = 0, br_lds_a = -1, br_lds_b = -1;
+    bool stage = false;
     size_t lds_bytes = 0;
 };
 
@@ -264,6 +265,11 @@ static void plan_small(Ctx &c, bool branch, int br_a, int br_b, SmallPlan &sp) {
     if (pb >= 0) sp.br_lds_b = buf_of[pb];
     sp.nbuf = (int)holder.size();
     sp.lds_bytes = small_lds_bytes(form, c.ncat, c.safe, c.su, sp.nbuf, (int)sp.tips.size());
+    // small op lists are staged in LDS with their P(t) exponentials (computed once per
+    // tile by all threads instead of once per lane of every op's wavefronts)
+    const size_t stage = small_stage_bytes(c.ncat, n, nlev);
+    sp.stage = n > 0 && stage <= 16 * 1024;
+    if (sp.stage) sp.lds_bytes = ((sp.lds_bytes + 15) & ~(size_t)15) + stage;
 }
 
 // Upload the queued ops, run the P(t) kernel, then one traversal launch in `mode`.
@@ -352,6 +358,7 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
             p.nbuf = sp.nbuf;
             p.br_lds_a = sp.br_lds_a;
             p.br_lds_b = sp.br_lds_b;
+            p.small_stage = sp.stage ? 1 : 0;
         }
         if (two_orders) {
             std::memcpy(h + ob, perm.data(), sizeof(int32_t) * nops);

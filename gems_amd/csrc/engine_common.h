@@ -97,6 +97,7 @@ struct KParams {
     int ntip;                 // tips whose states are staged in LDS
     const int32_t *tips;      // their taxon ids
     int br_lds_a, br_lds_b;   // LDS buffer (or staged tip index) of the branch's two sides
+    int small_stage;          // level-parallel kernel: op list, level bounds and P(t) exponentials staged in LDS
 };
 
 // Device-resident state of one Newton branch-length solve (Optimization::minimizeNewton,
@@ -182,6 +183,8 @@ hipError_t f32_theta(const KParams &p, hipStream_t s);
 // level-parallel 4-state traversal (form 0 FP64, 1 FP32), lds_bytes of dynamic LDS
 hipError_t small_traverse(const KParams &p, int form, int mode, int waves, size_t lds_bytes, hipStream_t s);
 size_t small_lds_bytes(int form, int ncat, int safe, int su, int nbuf, int ntip);
+// extra dynamic LDS of the level-parallel kernel's staged op list (0 = not staged)
+size_t small_stage_bytes(int ncat, int nops, int nlevels);
 
 // Shared host helpers (engine.cpp).
 KParams make_params(Ctx &c);

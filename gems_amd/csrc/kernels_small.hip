@@ -37,6 +37,13 @@ size_t small_lds_bytes(int form, int ncat, int safe, int su, int nbuf, int ntip)
     return off;
 }
 
+size_t small_stage_bytes(int ncat, int nops, int nlevels) {
+    // ops (48 B each), level bounds, then exp(lambda_i r_c t) per (op, side, category, i)
+    size_t b = (size_t)nops * sizeof(DevOp) + (size_t)(nlevels + 1) * 4;
+    b = (b + 15) & ~(size_t)15;
+    return b + (size_t)nops * 2 * ncat * 4 * 8;
+}
+
 hipError_t small_traverse(const KParams &p, int form, int mode, int slots, size_t lds, hipStream_t s) {
     if (mode != MODE_PARTIALS && mode != MODE_LOGL) return hipErrorInvalidValue;
     if (slots < 1 || slots > 4 || p.ncat > PLK_GPU_MAX_CATS) return hipErrorInvalidValue;
