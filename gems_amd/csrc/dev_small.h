@@ -175,6 +175,15 @@ __device__ __forceinline__ float side_f32(const double (&Pd)[16], int x, bool ti
     return a;
 }
 
+// Order this wavefront's LDS writes before its later reads (and reads before later
+// writes): one wavefront's LDS operations are performed in order, so only the compiler
+// has to be kept from moving them across.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------------------
 // FORM 0: FP64 eigen space, reference arithmetic (per category a[x] = eL[x].l,
 // b[x] = eR[x].r, out[j] = Uinv[j].(a*b); NORM / SAFE scale counts,
@@ -197,6 +206,9 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
     __shared__ float s_x[2][4][4][64];
     __shared__ double s_cat[PLK_GPU_MAX_CATS][64];
     __shared__ int s_sc[PLK_GPU_MAX_CATS][64];
+    // per wavefront: the two children's rows for the current op and category, formed
+    // once by lanes 0-31 (FP64: U[x][i] ex[i]; FP32: P(t)[x][y] in FP64)
+    __shared__ double s_rows[16][2][16];
 
     for (int i = threadIdx.x; i < (su + 1) * 4; i += blockDim.x)
         L.tipv[i] = FORM == 0 ? P.tip_lh[i] : P.tipvec[i];
@@ -234,28 +246,52 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
             REAL out[CPW][4];
             if (act) {
                 op = small_op(P, L, k);
-                const bool ltip = op.kind <= KIND_TI, rtip = op.kind == KIND_TT;
-                const int sl = ltip ? small_tip_state(P, L.state, op.left, op.lds_left, tile, lane) : 0;
-                const int sr = rtip ? small_tip_state(P, L.state, op.right, op.lds_right, tile, lane) : 0;
                 float flag = 1.0f;  // FP64 NORM: every value of my categories below 2^-256
                 float mx = 0.0f;    // FP32: maximum over my categories
+                // the op kind is uniform: each kind gets straight-line code
+                with_kind(op.kind, [&](auto KC) {
+                constexpr int KIND = decltype(KC)::value;
+                constexpr bool ltip = KIND <= KIND_TI, rtip = KIND == KIND_TT;
+                const int sl = ltip ? small_tip_state(P, L.state, op.left, op.lds_left, tile, lane) : 0;
+                const int sr = rtip ? small_tip_state(P, L.state, op.right, op.lds_right, tile, lane) : 0;
+                double (*rows)[16] = s_rows[w];
 #pragma unroll
                 for (int ci = 0; ci < CPW; ci++) {
                     const int c = cw + ci * CW;
                     if (c >= ncat) break;
+                    if (ci > 0) wave_lds_sync();  // the previous category's rows are read
+                    if (lane < 32) {
+                        const int sd = lane >> 4, x = (lane >> 2) & 3, i = lane & 3;
+                        double ex[4];
+                        small_exp_op(P, L, k, sd, c, sd ? op.len_right : op.len_left, ex);
+                        const cdptr U = as_const(P.evec);
+                        if constexpr (FORM == 0) {
+                            rows[sd][lane & 15] = U[x * 4 + i] * ex[i];
+                        } else {
+                            const cdptr Ui = as_const(P.inv_evec);
+                            double v = 0.0;  // P(t)[x][i], small_pmat's arithmetic
+#pragma unroll
+                            for (int j = 0; j < 4; j++) v = fma(U[x * 4 + j] * ex[j], Ui[j * 4 + i], v);
+                            rows[sd][lane & 15] = v;
+                        }
+                    }
                     REAL lv4[4] = {0, 0, 0, 0}, rv4[4] = {0, 0, 0, 0};
                     if (!ltip) small_child_vals<REAL>(P, L, op.left, op.lds_left, c, tile, lane, lv4);
                     if (!rtip) small_child_vals<REAL>(P, L, op.right, op.lds_right, c, tile, lane, rv4);
+                    wave_lds_sync();
                     if constexpr (FORM == 0) {
-                        double exl[4], exr[4];
-                        small_exp_op(P, L, k, 0, c, op.len_left, exl);
-                        small_exp_op(P, L, k, 1, c, op.len_right, exr);
+                        // side value (echild v)[x] with v the child or its tip vector; 1 for
+                        // the unknown state (the leaf table's row)
+                        auto side = [&](int sd, int x, bool tip, int st, const REAL(&v)[4]) {
+                            double a = 0.0;
+#pragma unroll
+                            for (int i = 0; i < 4; i++) a = fma(rows[sd][x * 4 + i], tip ? L.tipv[st * 4 + i] : v[i], a);
+                            return (tip && st == su) ? 1.0 : a;
+                        };
                         const cdptr uinv = as_const(P.inv_evec);
                         double t[4];
 #pragma unroll
-                        for (int x = 0; x < 4; x++)
-                            t[x] = side_f64(P, exl, x, ltip, sl, su, L.tipv, lv4) *
-                                   side_f64(P, exr, x, rtip, sr, su, L.tipv, rv4);
+                        for (int x = 0; x < 4; x++) t[x] = side(0, x, ltip, sl, lv4) * side(1, x, rtip, sr, rv4);
                         double m = 0.0;
 #pragma unroll
                         for (int j = 0; j < 4; j++) {
@@ -269,11 +305,11 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
                             // per-category check and counts (1997-2013)
                             const int64_t a = p * ncat + c;
                             unsigned sc = 0;
-                            if (op.kind == KIND_TI) sc = small_scale<double>(P, L, op.right, op.lds_right, c, lane, a);
-                            else if (op.kind == KIND_II)
+                            if (KIND == KIND_TI) sc = small_scale<double>(P, L, op.right, op.lds_right, c, lane, a);
+                            else if (KIND == KIND_II)
                                 sc = small_scale<double>(P, L, op.left, op.lds_left, c, lane, a) +
                                      small_scale<double>(P, L, op.right, op.lds_right, c, lane, a);
-                            if (op.kind != KIND_TT && m < kST && invar == 0.0) {
+                            if (KIND != KIND_TT && m < kST && invar == 0.0) {
 #pragma unroll
                                 for (int j = 0; j < 4; j++) out[ci][j] *= kTwo256;
                                 sc += 1;
@@ -284,19 +320,27 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
                             flag = 0.0f;
                         }
                     } else {
-                        double ex[4], PL[16], PR[16];
-                        small_exp_op(P, L, k, 0, c, op.len_left, ex);
-                        small_pmat(P, ex, PL);
-                        small_exp_op(P, L, k, 1, c, op.len_right, ex);
-                        small_pmat(P, ex, PR);
+                        // (P v)[x] in FP32; a tip's row in FP64, rounded once (the leaf table)
+                        auto side = [&](int sd, int x, bool tip, int st, const REAL(&v)[4]) {
+                            if (tip) {
+                                double wv = 0.0;
+#pragma unroll
+                                for (int y = 0; y < 4; y++) wv = fma(rows[sd][x * 4 + y], L.tipv[st * 4 + y], wv);
+                                return st == su ? 1.0f : (float)wv;
+                            }
+                            float a = 0.0f;
+#pragma unroll
+                            for (int y = 0; y < 4; y++) a = fmaf((float)rows[sd][x * 4 + y], v[y], a);
+                            return a;
+                        };
 #pragma unroll
                         for (int x = 0; x < 4; x++) {
-                            out[ci][x] = side_f32(PL, x, ltip, sl, su, L.tipv, lv4) *
-                                         side_f32(PR, x, rtip, sr, su, L.tipv, rv4);
+                            out[ci][x] = side(0, x, ltip, sl, lv4) * side(1, x, rtip, sr, rv4);
                             mx = fmaxf(mx, fabsf(out[ci][x]));
                         }
                     }
                 }
+                });
                 s_x[par][slot][cw][lane] = FORM == 0 ? flag : mx;
             }
             __syncthreads();

@@ -231,7 +231,10 @@ static void plan_small(Ctx &c, bool branch, int br_a, int br_b, SmallPlan &sp) {
     if (branch && br_a < 0) sp.br_lds_a = stage_tip(-br_a - 1);
     // 5. LDS buffers by liveness within the budget
     const int form = c.prec == PLK_FP64 ? 0 : 1;
-    const size_t budget = 48 * 1024;  // dynamic LDS; the kernel's own arrays take 14 KB
+    // dynamic LDS for partial buffers: up to 112 KB when every tile has a compute unit to
+    // itself (the kernel's own arrays take 24 KB, the staged op list up to 16 KB, of the
+    // 160 KB), else 32 KB so that two workgroups share one
+    const size_t budget = (c.ntiles <= 256 ? 112 : 32) * 1024;
     const size_t fixed = small_lds_bytes(form, c.ncat, c.safe, c.su, 0, (int)sp.tips.size());
     const size_t per_buf = small_lds_bytes(form, c.ncat, c.safe, c.su, 1, 0) - small_lds_bytes(form, c.ncat, c.safe, c.su, 0, 0);
     const int max_buf = budget > fixed ? (int)std::min<size_t>(64, (budget - fixed) / per_buf) : 0;
