@@ -191,7 +191,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 // then the pattern is renormalised to a maximum in [0.5, 1) and the exponents add up.
 // CPW = categories per wavefront (1 or 2).
 template <int FORM, int CPW, int MODE>
-__device__ inline void traverse_small_body(const KParams &P, const int tile, char *smem) {
+__device__ inline void traverse_small_body(const KParams &P, const int tile, char *smem, const uint32_t *inl) {
     using REAL = std::conditional_t<FORM == 0, double, float>;
     const SmallLds<REAL> L = small_lds<REAL>(P, smem);
     const int lane = threadIdx.x & 63;
@@ -212,16 +212,23 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
 
     for (int i = threadIdx.x; i < (su + 1) * 4; i += blockDim.x)
         L.tipv[i] = FORM == 0 ? P.tip_lh[i] : P.tipvec[i];
+    // the op list, level bounds and staged tip ids: in device memory, or (inl) in the
+    // kernel arguments as words [ops | level bounds | tip ids]
+    const int nw = P.nops * (int)(sizeof(DevOp) / 4);
+    const uint32_t *ops_src = inl ? inl : (const uint32_t *)P.ops;
+    const int32_t *lvl_src = inl ? (const int32_t *)inl + nw : P.lvl;
+    const int32_t *tips_src = inl ? (const int32_t *)inl + nw + P.nlevels + 1 : P.tips;
     for (int i = threadIdx.x; i < P.ntip * 64; i += blockDim.x)
-        L.state[i] = P.states[(int64_t)P.tips[i >> 6] * P.nptn_pad + (int64_t)tile * 64 + (i & 63)];
+        L.state[i] = P.states[(int64_t)tips_src[i >> 6] * P.nptn_pad + (int64_t)tile * 64 + (i & 63)];
     if (P.small_stage) {
-        const int nw = P.nops * (int)(sizeof(DevOp) / 4);
-        for (int i = threadIdx.x; i < nw; i += blockDim.x) L.ops_w[i] = ((const int32_t *)P.ops)[i];
-        for (int i = threadIdx.x; i <= P.nlevels; i += blockDim.x) L.lvl[i] = P.lvl[i];
+        for (int i = threadIdx.x; i < nw; i += blockDim.x) L.ops_w[i] = (int32_t)ops_src[i];
+        for (int i = threadIdx.x; i <= P.nlevels; i += blockDim.x) L.lvl[i] = lvl_src[i];
         const int ne = P.nops * 2 * ncat * 4;
         for (int idx = threadIdx.x; idx < ne; idx += blockDim.x) {
             const int i = idx & 3, c = (idx >> 2) % ncat, side = (idx / (4 * ncat)) & 1, k = idx / (8 * ncat);
-            const double t = side ? P.ops[k].len_right : P.ops[k].len_left;
+            // len_left / len_right: the op's words 8-9 / 10-11
+            const uint32_t *lw = ops_src + (size_t)k * (sizeof(DevOp) / 4) + 8 + 2 * side;
+            const double t = __builtin_bit_cast(double, ((unsigned long long)lw[1] << 32) | lw[0]);
             L.ex[idx] = exp(P.eval[i] * (P.rates[c] * t));
         }
     }

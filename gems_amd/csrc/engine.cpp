@@ -342,6 +342,8 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     if (small) c.jobs.clear();  // the level-parallel kernel forms P(t) itself
     std::vector<int32_t> perm;
     const bool two_orders = !small && alt_order(c, c.pending, perm);
+    bool inline_ops = false;
+    SmallInline inl;
     p.perm = nullptr;
     const int njobs = (int)c.jobs.size();
     if (njobs > 0 || small || two_orders) {
@@ -368,7 +370,15 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
             p.perm = (const int32_t *)((char *)c.d_ops + ob);
         }
         if (njobs > 0) std::memcpy(h + ob + xb, c.jobs.data(), jb);
-        if (ob + xb > 0) HIP_TRY(hipMemcpyAsync(c.d_ops, h, ob + xb, hipMemcpyHostToDevice, c.stream));
+        // a short staged list of the level-parallel kernel travels in its kernel arguments
+        inline_ops = small && sp.stage && njobs == 0 && (ob + xb) / 4 <= (size_t)kSmallInlineWords;
+        if (inline_ops) {
+            inl.nwords = (int32_t)((ob + xb) / 4);
+            inl.pad = 0;
+            std::memcpy(inl.data, h, ob + xb);
+        } else if (ob + xb > 0) {
+            HIP_TRY(hipMemcpyAsync(c.d_ops, h, ob + xb, hipMemcpyHostToDevice, c.stream));
+        }
         if (njobs > 0) {
             HIP_TRY(hipMemcpyAsync(c.d_jobs, h + ob + xb, jb, hipMemcpyHostToDevice, c.stream));
             timing_begin(c, 1);
@@ -380,7 +390,7 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     if (small) {
         const int smode = mode == MODE_LOGL ? MODE_LOGL : MODE_PARTIALS;
         timing_begin(c, 0);
-        HIP_TRY(small_traverse(p, f32 ? 1 : 0, smode, slots, sp.lds_bytes, c.stream));
+        HIP_TRY(small_traverse(p, f32 ? 1 : 0, smode, slots, sp.lds_bytes, inline_ops ? &inl : nullptr, c.stream));
         timing_end(c, 0);
         used[0] = true;
         if (mode == MODE_THETA) {
@@ -535,7 +545,8 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     c->d_tipvec = c->d_tip + (size_t)(c->su + 1) * n;
     c->d_pi = c->d_tipvec + (size_t)(c->su + 1) * n;
     c->d_piU = c->d_pi + n;
-    CREATE_TRY(hipHostMalloc((void **)&c->h_result, sizeof(double) * 2, hipHostMallocDefault));
+    CREATE_TRY(hipHostMalloc((void **)&c->h_result, sizeof(double) * 2, hipHostMallocMapped | hipHostMallocCoherent));
+    CREATE_TRY(hipHostGetDevicePointer((void **)&c->h_result_dev, c->h_result, 0));
     CREATE_TRY(hipHostMalloc((void **)&c->h_newton, sizeof(NewtonState), hipHostMallocDefault));
     for (int i = 0; i < 6; i++) CREATE_TRY(hipEventCreate(&c->ev[i]));
     // zero-initialised state: counters, scale counts and partials of unused slots
@@ -731,11 +742,12 @@ extern "C" int plk_branch_loglik(plk_ctx *c, int a, int b, double t, double *log
     p.br_a = a;
     p.br_b = b;
     p.br_len = t;
+    // the logL goes straight to the mapped host result (no copy after the launch)
+    p.result = c->h_result_dev;
     bool used[3] = {false, false, false};
     rc = run_traversal(*c, p, MODE_LOGL, used);
     if (rc) return rc;
     c->ptn_lh_valid = true;
-    HIP_TRY(hipMemcpyAsync(c->h_result, c->d_result, sizeof(double) * 2, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     timing_collect(*c, used);
     double v = c->h_result[0];

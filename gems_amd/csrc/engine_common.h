@@ -141,7 +141,8 @@ struct Ctx {
     // host staging (pinned)
     void *h_stage = nullptr;
     size_t h_stage_cap = 0;
-    double *h_result = nullptr;
+    double *h_result = nullptr;       // pinned, mapped: the branch logL kernel writes it directly
+    double *h_result_dev = nullptr;   // its device address
     std::vector<DevOp> pending;
     std::vector<PJob> jobs;
     int theta_a = 0, theta_b = 0;
@@ -181,7 +182,15 @@ hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_
 hipError_t f32_traverse(const KParams &p, int mode, hipStream_t s);
 hipError_t f32_theta(const KParams &p, hipStream_t s);
 // level-parallel 4-state traversal (form 0 FP64, 1 FP32), lds_bytes of dynamic LDS
-hipError_t small_traverse(const KParams &p, int form, int mode, int waves, size_t lds_bytes, hipStream_t s);
+// A short op list (ops, level bounds, staged tip ids, as 32-bit words) passed inside the
+// kernel arguments of the level-parallel kernel, so that no upload precedes the launch.
+constexpr int kSmallInlineWords = 720;
+struct SmallInline {
+    int32_t nwords, pad;
+    uint32_t data[kSmallInlineWords];
+};
+hipError_t small_traverse(const KParams &p, int form, int mode, int waves, size_t lds_bytes, const SmallInline *inl,
+                          hipStream_t s);
 size_t small_lds_bytes(int form, int ncat, int safe, int su, int nbuf, int ntip);
 // extra dynamic LDS of the level-parallel kernel's staged op list (0 = not staged)
 size_t small_stage_bytes(int ncat, int nops, int nlevels);

@@ -11,15 +11,27 @@ using namespace dev;
 template <int FORM, int CPW, int MODE>
 __global__ void __launch_bounds__(1024) k_traverse_small(KParams P) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    traverse_small_body<FORM, CPW, MODE>(P, blockIdx.x, smem);
+    traverse_small_body<FORM, CPW, MODE>(P, blockIdx.x, smem, nullptr);
+}
+
+// the same with the op list in the kernel arguments
+template <int FORM, int CPW, int MODE>
+__global__ void __launch_bounds__(1024) k_traverse_small_inl(KParams P, SmallInline in) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    traverse_small_body<FORM, CPW, MODE>(P, blockIdx.x, smem, in.data);
 }
 
 // slots = ops of a level run at once per tile, each by min(ncat, 4) wavefronts
 template <int FORM, int CPW>
-hipError_t launch_cpw(const KParams &p, int mode, int slots, size_t lds, hipStream_t s) {
+hipError_t launch_cpw(const KParams &p, int mode, int slots, size_t lds, const SmallInline *inl, hipStream_t s) {
     dim3 grid(p.ntiles), block(64 * std::min(p.ncat, 4) * slots);
-    if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_small<FORM, CPW, MODE_PARTIALS>), grid, block, lds, s, p);
-    else hipLaunchKernelGGL((k_traverse_small<FORM, CPW, MODE_LOGL>), grid, block, lds, s, p);
+    if (inl) {
+        if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_small_inl<FORM, CPW, MODE_PARTIALS>), grid, block, lds, s, p, *inl);
+        else hipLaunchKernelGGL((k_traverse_small_inl<FORM, CPW, MODE_LOGL>), grid, block, lds, s, p, *inl);
+    } else {
+        if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_small<FORM, CPW, MODE_PARTIALS>), grid, block, lds, s, p);
+        else hipLaunchKernelGGL((k_traverse_small<FORM, CPW, MODE_LOGL>), grid, block, lds, s, p);
+    }
     return hipGetLastError();
 }
 
@@ -44,11 +56,14 @@ size_t small_stage_bytes(int ncat, int nops, int nlevels) {
     return b + (size_t)nops * 2 * ncat * 4 * 8;
 }
 
-hipError_t small_traverse(const KParams &p, int form, int mode, int slots, size_t lds, hipStream_t s) {
+hipError_t small_traverse(const KParams &p, int form, int mode, int slots, size_t lds, const SmallInline *inl,
+                          hipStream_t s) {
     if (mode != MODE_PARTIALS && mode != MODE_LOGL) return hipErrorInvalidValue;
     if (slots < 1 || slots > 4 || p.ncat > PLK_GPU_MAX_CATS) return hipErrorInvalidValue;
-    if (form == 0) return p.ncat <= 4 ? launch_cpw<0, 1>(p, mode, slots, lds, s) : launch_cpw<0, 2>(p, mode, slots, lds, s);
-    return p.ncat <= 4 ? launch_cpw<1, 1>(p, mode, slots, lds, s) : launch_cpw<1, 2>(p, mode, slots, lds, s);
+    if (inl && !p.small_stage) return hipErrorInvalidValue;  // inline lists are read once, into LDS
+    if (form == 0)
+        return p.ncat <= 4 ? launch_cpw<0, 1>(p, mode, slots, lds, inl, s) : launch_cpw<0, 2>(p, mode, slots, lds, inl, s);
+    return p.ncat <= 4 ? launch_cpw<1, 1>(p, mode, slots, lds, inl, s) : launch_cpw<1, 2>(p, mode, slots, lds, inl, s);
 }
 
 }  // namespace plk
