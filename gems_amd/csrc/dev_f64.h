@@ -17,7 +17,8 @@ namespace dev {
 
 // ---------------------------------------------------------------------------------------
 // P(t) kernel (computePartialInfo, tree/phylokernelnew.h:1013-1063): for one child of one
-// op, echild[c][x][i] = U[x][i] exp(lambda_i r_c t), and for a leaf child the table
+// op, echild[c][x][i] = U[x][i] exp(lambda_i r_c t) (held as [c][i][x] for n <= 20), and
+// for a leaf child the table
 // leaf[s][c][x] = sum_i echild[c][x][i] tip_lh[s][i] with the unknown-state row = 1.
 __device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j) {
     const int n = P.n, ncat = P.ncat, block = n * ncat, su = P.su;
@@ -29,9 +30,11 @@ __device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j) {
         ex[idx] = exp(P.eval[i] * (P.rates[c] * j.len));
     }
     __syncthreads();
+    // stored [c][i][x] (a column per i) for n <= 20, whose kernels read a column at a time
+    const bool cols = n <= 20;
     for (int idx = threadIdx.x; idx < ncat * n * n; idx += blockDim.x) {
         int c = idx / (n * n), rem = idx % (n * n), x = rem / n, i = rem % n;
-        E[idx] = P.evec[x * n + i] * ex[c * n + i];
+        E[cols ? (int64_t)c * n * n + i * n + x : idx] = P.evec[x * n + i] * ex[c * n + i];
     }
     if (j.leaf) {
         double *T = M + (int64_t)2 * ncat * n * n + (int64_t)j.side * (su + 1) * block;
@@ -106,20 +109,89 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
     }
 }
 
-// Same arithmetic as op_partial for N <= 20, phased so that fewer registers are live:
-// (1) b[x] = eR[x].R, (2) t[x] = (eL[x].L) * b[x] in place, (3) out[j] = Uinv[j].t is
-// stored as soon as it is formed.  Every sum keeps op_partial's order, so the stored
-// values are identical.  Returns max_j |out[j]| (the scaling check is applied to the
-// stored values by the caller).
+// A column of K (2, 4, 10 or 20) wave-uniform doubles read into scalar registers by
+// back-to-back scalar loads that are waited on once.  Left to itself the compiler streams
+// such values through a single 16-register tuple, one memory round trip per 8 values;
+// here a whole column costs one.  The loads and their wait sit in one asm statement, so
+// no register is read before its data has arrived; the outputs are early-clobber, since
+// later loads of the statement still read the base address while earlier ones are
+// writing their destination registers.  Callers fence each column's FMAs
+// with scheduling barriers (one column live at a time) and pass an opaque base, so the
+// column addresses are not hoisted into registers for the whole loop.
+typedef unsigned su4 __attribute__((ext_vector_type(4)));
+typedef unsigned su8 __attribute__((ext_vector_type(8)));
+typedef unsigned su16 __attribute__((ext_vector_type(16)));
+
+template <int K>
+struct SCol;
+template <>
+struct SCol<20> {
+    su16 a, b;
+    su8 c;
+    __device__ __forceinline__ SCol(const double *p) {
+        asm volatile("s_load_dwordx16 %0, %3, 0x0\n\ts_load_dwordx16 %1, %3, 0x40\n\t"
+                     "s_load_dwordx8 %2, %3, 0x80\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&s"(a), "=&s"(b), "=&s"(c)
+                     : "s"(p));
+    }
+    __device__ __forceinline__ double operator[](int i) const {
+        const unsigned lo = i < 8 ? a[2 * i] : i < 16 ? b[2 * i - 16] : c[2 * i - 32];
+        const unsigned hi = i < 8 ? a[2 * i + 1] : i < 16 ? b[2 * i - 15] : c[2 * i - 31];
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    }
+};
+template <>
+struct SCol<10> {
+    su16 a;
+    su4 b;
+    __device__ __forceinline__ SCol(const double *p) {
+        asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx4 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+                     : "=&s"(a), "=&s"(b)
+                     : "s"(p));
+    }
+    __device__ __forceinline__ double operator[](int i) const {
+        const unsigned lo = i < 8 ? a[2 * i] : b[2 * i - 16];
+        const unsigned hi = i < 8 ? a[2 * i + 1] : b[2 * i - 15];
+        return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+    }
+};
+template <>
+struct SCol<4> {
+    su8 a;
+    __device__ __forceinline__ SCol(const double *p) {
+        asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=&s"(a) : "s"(p));
+    }
+    __device__ __forceinline__ double operator[](int i) const {
+        return __builtin_bit_cast(double, ((unsigned long long)a[2 * i + 1] << 32) | a[2 * i]);
+    }
+};
+template <>
+struct SCol<2> {
+    su4 a;
+    __device__ __forceinline__ SCol(const double *p) {
+        asm volatile("s_load_dwordx4 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=&s"(a) : "s"(p));
+    }
+    __device__ __forceinline__ double operator[](int i) const {
+        return __builtin_bit_cast(double, ((unsigned long long)a[2 * i + 1] << 32) | a[2 * i]);
+    }
+};
+
+// Same arithmetic as op_partial for N <= 20: echild is held column-major ([i][x]), so
+// each step of (1) t[x] = eR[x].R and (2) t[x] *= eL[x].L (in two halves of x) reads one
+// contiguous column of wave-uniform values (SCol) and feeds independent FMAs; (3) out[j] =
+// Uinv[j].t reads U^-1 a row at a time and is stored as formed.  Every sum runs over its index in
+// ascending order from 0, as in op_partial, so the stored values are identical.  Returns
+// max_j |out[j]| (the scaling check is applied to the stored values by the caller).
 template <int N>
 __device__ __forceinline__ double op_partial_store(const KParams &P, const DevOp &op, const double *__restrict__ M,
                                                    int c, int64_t off, int sl, int sr, double *__restrict__ dst) {
-    static_assert(N <= 20, "phased form is for unrolled state loops");
+    static_assert(N <= 20, "column form is for unrolled state loops");
     const int ncat = P.ncat, block = N * ncat;
     const double *__restrict__ part = (const double *)P.part;
-    const cdptr uinv = as_const(opaque(P.inv_evec));
-    const cdptr eL = as_const(M + (int64_t)c * N * N);
-    const cdptr eR = as_const(M + (int64_t)(ncat + c) * N * N);
+    static_assert(N % 2 == 0, "columns are split in two equal halves");
+    const double *uinv = P.inv_evec;
+    const double *eL = M + (int64_t)c * N * N;
+    const double *eR = M + (int64_t)(ncat + c) * N * N;
     const double *__restrict__ tabs = M + (int64_t)2 * ncat * N * N;
     double t[N];
     if (op.kind == KIND_TT) {
@@ -133,44 +205,58 @@ __device__ __forceinline__ double op_partial_store(const KParams &P, const DevOp
             double r[N];
 #pragma unroll
             for (int i = 0; i < N; i++) r[i] = R[i * 64];
-            static_for<0, N>([&](auto X) {
-                constexpr int x = decltype(X)::value;
-                double b = 0.0;
 #pragma unroll
-                for (int i = 0; i < N; i++) b = fma(eR[x * N + i], r[i], b);
-                t[x] = b;
-            });
+            for (int x = 0; x < N; x++) t[x] = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const SCol<N> col(opaque(eR) + i * N);
+#pragma unroll
+                for (int x = 0; x < N; x++) t[x] = fma(col[x], r[i], t[x]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         // keep the left child's loads behind phase (1): the right child's registers are
-        // free by then, which keeps the 20-state kernel at four waves per SIMD
+        // free by then
         __builtin_amdgcn_sched_barrier(0);
         if (op.kind == KIND_II) {
             const double *__restrict__ L = part + (int64_t)op.left * P.slot_stride + off;
             double l[N];
 #pragma unroll
             for (int i = 0; i < N; i++) l[i] = L[i * 64];
-            static_for<0, N>([&](auto X) {
-                constexpr int x = decltype(X)::value;
-                double a = 0.0;
+            constexpr int H = N / 2;
 #pragma unroll
-                for (int i = 0; i < N; i++) a = fma(eL[x * N + i], l[i], a);
-                t[x] = a * t[x];
-            });
+            for (int h = 0; h < N; h += H) {
+                double a[H];
+#pragma unroll
+                for (int x = 0; x < H; x++) a[x] = 0.0;
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    const SCol<H> col(opaque(eL) + i * N + h);
+#pragma unroll
+                    for (int x = 0; x < H; x++) a[x] = fma(col[x], l[i], a[x]);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll
+                for (int x = 0; x < H; x++) t[h + x] = a[x] * t[h + x];
+            }
         } else {
             const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
 #pragma unroll
             for (int x = 0; x < N; x++) t[x] = tl[x] * t[x];
         }
     }
+    // (3) out[j] = Uinv[j].t, a row of U^-1 at a time (one accumulator per output)
     double m = 0.0;
-    static_for<0, N>([&](auto J) {
-        constexpr int j = decltype(J)::value;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const SCol<N> row(opaque(uinv) + j * N);
         double o = 0.0;
 #pragma unroll
-        for (int x = 0; x < N; x++) o = fma(uinv[j * N + x], t[x], o);
+        for (int x = 0; x < N; x++) o = fma(row[x], t[x], o);
         m = fmax(m, fabs(o));
         dst[j * 64] = o;
-    });
+        __builtin_amdgcn_sched_barrier(0);
+    }
     return m;
 }
 
