@@ -135,6 +135,10 @@ _SIGS = {
     "plk_branch_derivs_multi": (_int, [_P, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp, _dp, _dp]),
     "plk_tree_autotune": (_int, [_P, _int, ctypes.POINTER(_int), _dp]),
     "plk_tree_queue_branch": (_int, [_P, _int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp]),
+    "plk_tree_set_gamma_shape": (_int, [_P, _dbl]),
+    "plk_tree_get_gamma_shape": (_int, [_P, _dp]),
+    "plk_tree_optimize_gamma": (_int, [_P, _dbl, _dbl, _dbl, _dp, _dp]),
+    "plk_tree_optimize_parameters": (_int, [_P, _int, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _dp, _dp]),
     "plk_sim_create_on_tree": (_int, [ctypes.POINTER(_P), _int, _int, _ip, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
     # batched partitions
     "plk_batch_create": (_int, [ctypes.POINTER(_P), ctypes.POINTER(_P), _int]),
@@ -280,6 +284,12 @@ class Tree:
         _check(lib().plk_tree_loglik(self.h, a, b, ctypes.byref(v), _d(pl) if pl is not None else None))
         return (v.value, pl) if with_patterns else v.value
 
+    def compute_likelihood(self):
+        """computeLikelihood(): logL at the first leaf's branch, recomputing only dirty partials."""
+        v = ctypes.c_double()
+        _check(lib().plk_tree_compute_likelihood(self.h, ctypes.byref(v)))
+        return v.value
+
     def full_traversal(self):
         v = ctypes.c_double()
         _check(lib().plk_tree_full_traversal(self.h, ctypes.byref(v)))
@@ -344,6 +354,29 @@ class Tree:
         v = ctypes.c_double()
         _check(lib().plk_tree_optimize_all_branches(self.h, iterations, tol, min_bl, max_bl, max_nr, ctypes.byref(v)))
         return v.value
+
+    def set_gamma_shape(self, alpha):
+        _check(lib().plk_tree_set_gamma_shape(self.h, alpha))
+
+    def gamma_shape(self):
+        v = ctypes.c_double()
+        _check(lib().plk_tree_get_gamma_shape(self.h, ctypes.byref(v)))
+        return v.value
+
+    def optimize_gamma(self, gradient_epsilon=1e-4, min_gamma=0.02, max_gamma=1000.0):
+        """RateGamma::optimizeParameters; returns (alpha, logL)."""
+        a, v = ctypes.c_double(), ctypes.c_double()
+        _check(lib().plk_tree_optimize_gamma(self.h, gradient_epsilon, min_gamma, max_gamma, ctypes.byref(a),
+                                             ctypes.byref(v)))
+        return a.value, v.value
+
+    def optimize_parameters(self, num_iterations=100, logl_epsilon=0.01, gradient_epsilon=1e-4, min_gamma=0.02,
+                            max_gamma=1000.0, min_bl=1e-6, max_bl=10.0, max_nr=100):
+        """ModelFactory::optimizeParameters (branch lengths + Gamma shape); returns (alpha, logL)."""
+        a, v = ctypes.c_double(), ctypes.c_double()
+        _check(lib().plk_tree_optimize_parameters(self.h, num_iterations, logl_epsilon, gradient_epsilon, min_gamma,
+                                                  max_gamma, min_bl, max_bl, max_nr, ctypes.byref(a), ctypes.byref(v)))
+        return a.value, v.value
 
     def counters(self):
         out = np.zeros(3, dtype=np.int64)

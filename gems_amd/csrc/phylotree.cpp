@@ -4,8 +4,10 @@
 #include <cmath>
 #include <vector>
 
+#include "brent.h"
 #include "plk.h"
 #include "plk_internal.h"
+#include "plk_model.h"
 #include "plk_tree.h"
 
 namespace plk {
@@ -30,6 +32,12 @@ class PhyloTreeGPU {
     bool device_newton = true;
     // batched NNI: scratch slots start at nni_base, 3 per candidate, nni_cands candidates
     int nni_base = 0, nni_cands = 0;
+    // The model as last set, kept so that a new Gamma shape re-uploads it with new rates.
+    // gamma_shape is 0 while the rates are the caller's own (not derived from a shape).
+    int nstates = 0, ncat = 0, su = 0;
+    std::vector<double> m_eval, m_evec, m_inv, m_rates, m_props, m_tip;
+    bool model_stored = false;
+    double gamma_shape = 0.0;
 
     bool is_leaf(int v) const { return v < ntaxa; }
 
@@ -534,6 +542,80 @@ class PhyloTreeGPU {
         out = tree_lh;
         return PLK_OK;
     }
+
+    int upload_model() {
+        clear_all();
+        return plk_set_model(ctx, m_eval.data(), m_evec.data(), m_inv.data(), m_rates.data(), m_props.data(), su,
+                             m_tip.data());
+    }
+
+    // RateGamma::setGammaShape -> computeRates (model/rategamma.cpp:173-176, 98-171), then
+    // clearAllPartialLH: every partial depends on the category rates.
+    int set_gamma(double alpha) {
+        if (!model_stored) return fail(PLK_ERR_STATE, "Gamma shape needs a model (plk_tree_set_model) first");
+        std::vector<double> r(ncat);
+        int rc = plk_gamma_rates(alpha, ncat, r.data());
+        if (rc) return rc;
+        m_rates = r;
+        gamma_shape = alpha;
+        return upload_model();
+    }
+
+    // RateGamma::computeFunction (model/rategamma.cpp:178-185): -logL at shape; the
+    // partials are recomputed only when the shape changes.
+    int gamma_function(double shape, double &f) {
+        int rc;
+        if (shape != gamma_shape && (rc = set_gamma(shape))) return rc;
+        double lnl = 0.0;
+        rc = loglik(0, nei[0], &lnl, nullptr);
+        f = -lnl;
+        return rc;
+    }
+
+    // RateGamma::optimizeParameters(gradient_epsilon, min_gamma, max_gamma)
+    // (model/rategamma.cpp:215-232): minimizeOneDimen over the shape with tolerance
+    // max(gradient_epsilon, TOL_GAMMA_SHAPE = 0.001) (utils/tools.h:607); the model is left
+    // at the returned shape and its logL is returned.
+    int optimize_gamma(double gradient_epsilon, double min_gamma, double max_gamma, double &logl) {
+        if (!(gamma_shape > 0.0)) return fail(PLK_ERR_STATE, "plk_tree_optimize_gamma: no Gamma shape set");
+        auto f = [this](double x, double &fx) { return gamma_function(x, fx); };
+        double xopt = 0.0, fopt = 0.0, f2 = 0.0;
+        int rc = minimize_one_dimen(f, min_gamma, gamma_shape, max_gamma, std::max(gradient_epsilon, 0.001), xopt, fopt,
+                                    f2);
+        if (rc) return rc;
+        double fx = 0.0;
+        rc = gamma_function(xopt, fx);
+        logl = -fx;
+        return rc;
+    }
+
+    // ModelFactory::optimizeParameters (model/modelfactory.cpp:1558-1720) with branch
+    // lengths optimised (BRLEN_OPTIMIZE), a substitution model with nothing free
+    // (ModelSubst::optimizeParameters adds nothing: optimizeParametersOnly returns the rate
+    // model's logL, 1263-1315) and a Gamma rate model: alternate all-branch Newton passes
+    // (min(i, 3) rounds) with the shape until an iteration gains no more than
+    // logl_epsilon, then a final optimizeAllBranches(100).  RateGamma::rescaleRates is the
+    // base class's 1.0 (model/rateheterogeneity.h:288), so no length rescaling follows.
+    int optimize_parameters(int num_iterations, double logl_epsilon, double gradient_epsilon, double min_gamma,
+                            double max_gamma, double min_bl, double max_bl, int max_nr, double &out) {
+        if (!(gamma_shape > 0.0)) return fail(PLK_ERR_STATE, "plk_tree_optimize_parameters: no Gamma shape set");
+        double cur = 0.0;
+        int rc = loglik(0, nei[0], &cur, nullptr);
+        if (rc) return rc;
+        for (int i = 2; i < num_iterations; i++) {
+            double new_lh = 0.0;
+            if ((rc = optimize_all(std::min(i, 3), logl_epsilon, min_bl, max_bl, max_nr, new_lh))) return rc;
+            if ((rc = optimize_gamma(gradient_epsilon, min_gamma, max_gamma, new_lh))) return rc;
+            if (new_lh > cur + logl_epsilon) {
+                cur = new_lh;
+            } else {
+                if ((rc = optimize_all(100, logl_epsilon, min_bl, max_bl, max_nr, cur))) return rc;
+                break;
+            }
+        }
+        out = cur;
+        return PLK_OK;
+    }
 };
 
 }  // namespace plk
@@ -553,6 +635,8 @@ extern "C" int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int nta
     t->all_dirs = all_directions != 0;
     t->nni_base = 3 * (ntaxa - 2);
     t->nni_cands = all_directions == 2 ? 2 * (ntaxa - 3) : 0;
+    t->nstates = nstates;
+    t->ncat = ncat;
     t->nei.assign((size_t)t->nnodes * 3, -1);
     t->len.assign((size_t)t->nnodes * 3, 0.0);
     t->orient.assign(t->nnodes, -1);
@@ -585,7 +669,55 @@ extern "C" int plk_tree_set_model(plk_tree *t, const double *eval, const double 
                                   const double *rates, const double *props, int su, const double *tip_lh) {
     if (!t) return fail(PLK_ERR_ARG, "null tree");
     t->clear_all();
-    return plk_set_model(t->ctx, eval, evec, inv_evec, rates, props, su, tip_lh);
+    int rc = plk_set_model(t->ctx, eval, evec, inv_evec, rates, props, su, tip_lh);
+    if (rc) return rc;
+    const size_t n = (size_t)t->nstates, nc = (size_t)t->ncat;
+    t->su = su;
+    t->m_eval.assign(eval, eval + n);
+    t->m_evec.assign(evec, evec + n * n);
+    t->m_inv.assign(inv_evec, inv_evec + n * n);
+    t->m_rates.assign(rates, rates + nc);
+    t->m_props.assign(props, props + nc);
+    t->m_tip.assign(tip_lh, tip_lh + (size_t)(su + 1) * n);
+    t->model_stored = true;
+    t->gamma_shape = 0.0;
+    return PLK_OK;
+}
+
+extern "C" int plk_tree_set_gamma_shape(plk_tree *t, double alpha) {
+    if (!t || !(alpha > 0.0) || !std::isfinite(alpha)) return fail(PLK_ERR_ARG, "plk_tree_set_gamma_shape: bad arguments");
+    return t->set_gamma(alpha);
+}
+
+extern "C" int plk_tree_get_gamma_shape(plk_tree *t, double *alpha) {
+    if (!t || !alpha) return fail(PLK_ERR_ARG, "plk_tree_get_gamma_shape: bad arguments");
+    *alpha = t->gamma_shape;
+    return PLK_OK;
+}
+
+static bool gamma_bounds_ok(double min_gamma, double max_gamma) {
+    return min_gamma > 0.0 && max_gamma > min_gamma && std::isfinite(max_gamma);
+}
+
+extern "C" int plk_tree_optimize_gamma(plk_tree *t, double gradient_epsilon, double min_gamma, double max_gamma,
+                                       double *alpha, double *logl) {
+    if (!t || !logl || !(gradient_epsilon >= 0.0) || !gamma_bounds_ok(min_gamma, max_gamma))
+        return fail(PLK_ERR_ARG, "plk_tree_optimize_gamma: bad arguments");
+    int rc = t->optimize_gamma(gradient_epsilon, min_gamma, max_gamma, *logl);
+    if (alpha) *alpha = t->gamma_shape;
+    return rc;
+}
+
+extern "C" int plk_tree_optimize_parameters(plk_tree *t, int num_param_iterations, double logl_epsilon,
+                                            double gradient_epsilon, double min_gamma, double max_gamma, double min_bl,
+                                            double max_bl, int max_nr_step, double *alpha, double *logl) {
+    if (!t || !logl || num_param_iterations < 0 || !(logl_epsilon >= 0.0) || !(gradient_epsilon >= 0.0) ||
+        !gamma_bounds_ok(min_gamma, max_gamma) || !(min_bl >= 0.0) || !(max_bl > min_bl) || max_nr_step < 0)
+        return fail(PLK_ERR_ARG, "plk_tree_optimize_parameters: bad arguments");
+    int rc = t->optimize_parameters(num_param_iterations, logl_epsilon, gradient_epsilon, min_gamma, max_gamma, min_bl,
+                                    max_bl, max_nr_step, *logl);
+    if (alpha) *alpha = t->gamma_shape;
+    return rc;
 }
 
 extern "C" int plk_tree_set_patterns(plk_tree *t, const uint8_t *states, const double *freq, const double *invar) {

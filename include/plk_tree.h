@@ -84,6 +84,38 @@ int plk_tree_all_branch_derivs(plk_tree *t, int32_t *branch_a, int32_t *branch_b
 /* Counters: [0] partial updates (site x node), [1] derivative evaluations, [2] traversal ops */
 int plk_tree_counters(plk_tree *t, int64_t *out3);
 
+/* ---- model-parameter loop ------------------------------------------------------------
+ * The rate model is discrete Gamma with ncat mean-rate categories; the substitution model
+ * (eigen system, frequencies) stays as plk_tree_set_model gave it.  The tree keeps a copy
+ * of that model and re-uploads it with the new category rates whenever the shape changes
+ * (all partials become dirty).  logL values are computeLikelihood()'s (leaf 0's branch). */
+
+/* RateGamma::setGammaShape + computeRates (model/rategamma.cpp:173-176, 98-171): rates of
+ * the plk_gamma_rates categories of shape alpha replace the model's rates (props are
+ * kept).  PLK_ERR_STATE before plk_tree_set_model; plk_tree_set_model itself clears the
+ * shape (rates are then the caller's own). */
+int plk_tree_set_gamma_shape(plk_tree *t, double alpha);
+/* current shape, 0 when none is set */
+int plk_tree_get_gamma_shape(plk_tree *t, double *alpha);
+/* RateGamma::optimizeParameters(gradient_epsilon, min_gamma, max_gamma)
+ * (model/rategamma.cpp:215-232): Brent's method behind Optimization::minimizeOneDimen
+ * (utils/optimization.cpp:174-372) over the shape, starting at the current one, with
+ * tolerance max(gradient_epsilon, TOL_GAMMA_SHAPE = 0.001) (utils/tools.h:605-607 also
+ * gives the reference's bounds MIN_GAMMA_SHAPE 0.02, MAX_GAMMA_SHAPE 1000).  Each
+ * evaluation is a full traversal on the device.  Leaves the model at the optimum; alpha
+ * (nullable) and logl receive it and its logL.  PLK_ERR_STATE without a shape. */
+int plk_tree_optimize_gamma(plk_tree *t, double gradient_epsilon, double min_gamma, double max_gamma, double *alpha,
+                            double *logl);
+/* ModelFactory::optimizeParameters(BRLEN_OPTIMIZE, ..., logl_epsilon, gradient_epsilon)
+ * (model/modelfactory.cpp:1558-1720) for a substitution model with no free parameters
+ * and a Gamma rate model: for i = 2 .. num_param_iterations-1, optimizeAllBranches(min(i,
+ * 3), logl_epsilon) then the shape (plk_tree_optimize_gamma); stop when an iteration's
+ * logL is not above the previous one + logl_epsilon, then optimizeAllBranches(100,
+ * logl_epsilon).  The reference's defaults: num_param_iterations 100, max_nr_step 100. */
+int plk_tree_optimize_parameters(plk_tree *t, int num_param_iterations, double logl_epsilon, double gradient_epsilon,
+                                 double min_gamma, double max_gamma, double min_bl, double max_bl, int max_nr_step,
+                                 double *alpha, double *logl);
+
 #ifdef __cplusplus
 }
 #endif

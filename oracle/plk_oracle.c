@@ -51,6 +51,7 @@ struct orc_tree {
     double *mats;              /* scratch for echildren/leaf tables of the current traversal */
     size_t mats_cap;
     long long updates;
+    double gamma_shape;        /* 0: rates are the caller's own */
 };
 
 static void *xalloc(size_t bytes) {
@@ -145,6 +146,7 @@ void orc_set_model(orc_tree *t, const double *eval, const double *evec, const do
     memcpy(t->inv_evec, inv_evec, sizeof(double) * n * n);
     memcpy(t->rates, rates, sizeof(double) * t->ncat);
     memcpy(t->props, props, sizeof(double) * t->ncat);
+    t->gamma_shape = 0.0;
     orc_clear_all(t);
 }
 
@@ -977,4 +979,257 @@ double orc_nni_eval(orc_tree *t, int node1, int node2, int move, double min_bl, 
     free(nei0);
     free(len0);
     return score;
+}
+
+/* ---- model-parameter loop ---------------------------------------------------------- */
+
+/* Discrete-Gamma mean rates (RateGamma::computeRatesMean, model/rategamma.cpp:98-171).
+ * The reference takes the category boundaries from published algorithms; restated from
+ * their papers: log Gamma by the Stirling series after shifting the argument to >= 7
+ * (Pike & Hill 1966), the incomplete Gamma ratio by series or continued fraction
+ * (Bhattacharjee 1970, AS 32), normal quantiles by a rational approximation (Odeh & Evans
+ * 1974), chi-square quantiles by a starting approximation refined with a seventh-order
+ * Taylor series (Best & Roberts 1975, AS 91).  Their stopping rules set the accuracy
+ * (~1e-6 relative); the rates must carry the same rounding as the reference's for a
+ * shape search to take the same steps. */
+static double orc_lgamma(double x) {
+    double shift = 0.0;
+    if (x < 7.0) {
+        /* Gamma(x) = Gamma(x + m) / (x (x+1) ... (x+m-1)), x + m the first value >= 7 */
+        double prod = 1.0;
+        for (; x < 7.0; x += 1.0) prod *= x;
+        shift = -log(prod);
+    }
+    const double z = 1.0 / (x * x);
+    const double series = (((-0.000595238095238 * z + 0.000793650793651) * z - 0.002777777777778) * z +
+                           0.083333333333333) / x;
+    return shift + (x - 0.5) * log(x) - x + 0.918938533204673 + series;
+}
+
+static double orc_incgamma(double x, double p, double lgp) {
+    const double acc = 1e-8, big = 1e30;
+    if (x == 0.0) return 0.0;
+    if (x < 0.0 || p <= 0.0) return -1.0;
+    const double lead = exp(p * log(x) - x - lgp);
+    if (x <= 1.0 || x < p) {
+        double sum = 1.0, term = 1.0, den = p;
+        do {
+            den += 1.0;
+            term *= x / den;
+            sum += term;
+        } while (term > acc);
+        return sum * lead / p;
+    }
+    double a = 1.0 - p, b = a + x + 1.0, k = 0.0;
+    double A0 = 1.0, B0 = x, A1 = x + 1.0, B1 = x * b;
+    double cf = A1 / B1;
+    for (;;) {
+        a += 1.0;
+        b += 2.0;
+        k += 1.0;
+        const double an = a * k;
+        const double A2 = b * A1 - an * A0, B2 = b * B1 - an * B0;
+        if (B2 != 0.0) {
+            const double r = A2 / B2, d = fabs(cf - r);
+            if (d <= acc && d <= acc * r) return 1.0 - lead * cf;
+            cf = r;
+        }
+        A0 = A1; B0 = B1; A1 = A2; B1 = B2;
+        if (fabs(A1) >= big) { A0 /= big; B0 /= big; A1 /= big; B1 /= big; }
+    }
+}
+
+static double orc_norm_quantile(double prob) {
+    const double p1 = prob < 0.5 ? prob : 1.0 - prob;
+    if (p1 < 1e-20) return -9999.0;
+    const double y = sqrt(log(1.0 / (p1 * p1)));
+    const double num = (((y * -0.453642210148e-4 - 0.0204231210245) * y - 0.342242088547) * y - 1.0) * y -
+                       0.322232431088;
+    const double den = (((y * 0.0038560700634 + 0.103537752850) * y + 0.531103462366) * y + 0.588581570495) * y +
+                       0.0993484626060;
+    const double z = y + num / den;
+    return prob < 0.5 ? -z : z;
+}
+
+static double orc_chi2_quantile(double prob, double v) {
+    const double e = 0.5e-6, ln2 = 0.6931471805;
+    if (prob < 0.000002 || prob > 0.999998 || v <= 0.0) return -1.0;
+    const double g = orc_lgamma(v / 2.0), xx = v / 2.0, c = xx - 1.0;
+    double ch;
+    if (v < -1.24 * log(prob)) {
+        ch = pow(prob * xx * exp(g + xx * ln2), 1.0 / xx);
+        if (ch - e < 0.0) return ch;
+    } else if (v <= 0.32) {
+        const double la = log(1.0 - prob);
+        double prev;
+        ch = 0.4;
+        do {
+            prev = ch;
+            const double q1 = 1.0 + ch * (4.67 + ch);
+            const double q2 = ch * (6.73 + ch * (6.66 + ch));
+            const double tt = -0.5 + (4.67 + 2.0 * ch) / q1 - (6.73 + ch * (13.32 + 3.0 * ch)) / q2;
+            ch -= (1.0 - exp(la + g + 0.5 * ch + c * ln2) * q2 / q1) / tt;
+        } while (fabs(prev / ch - 1.0) - 0.01 > 0.0);
+    } else {
+        const double zq = orc_norm_quantile(prob), w = 0.222222 / v;
+        ch = v * pow(zq * sqrt(w) + 1.0 - w, 3.0);
+        if (ch > 2.2 * v + 6.0) ch = -2.0 * (log(1.0 - prob) - c * log(0.5 * ch) + g);
+    }
+    double prev;
+    do {
+        prev = ch;
+        const double half = 0.5 * ch;
+        double tt = orc_incgamma(half, xx, g);
+        if (tt < 0.0) return -1.0;
+        tt = (prob - tt) * exp(xx * ln2 + g + half - c * log(ch));
+        const double b = tt / ch, a = 0.5 * tt - b * c;
+        const double s1 = (210 + a * (140 + a * (105 + a * (84 + a * (70 + 60 * a))))) / 420;
+        const double s2 = (420 + a * (735 + a * (966 + a * (1141 + 1278 * a)))) / 2520;
+        const double s3 = (210 + a * (462 + a * (707 + 932 * a))) / 2520;
+        const double s4 = (252 + a * (672 + 1182 * a) + c * (294 + a * (889 + 1740 * a))) / 5040;
+        const double s5 = (84 + 264 * a + c * (175 + 606 * a)) / 2520;
+        const double s6 = (120 + c * (346 + 127 * c)) / 5040;
+        ch += tt * (1 + 0.5 * tt * s1 - b * c * (s1 - b * (s2 - b * (s3 - b * (s4 - b * (s5 - b * s6))))));
+    } while (fabs(prev / ch - 1.0) > e);
+    return ch;
+}
+
+void orc_gamma_rates(double alpha, int ncat, double *rates) {
+    if (ncat == 1) { rates[0] = 1.0; return; }
+    /* cut points of Gamma(alpha, rate alpha) = chi2(2 alpha) / (2 alpha); the mean rate of a
+     * category is ncat x (mass of Gamma(alpha + 1) between its cuts) */
+    const double lg1 = orc_lgamma(alpha + 1.0);
+    double prev = 0.0, sum = 0.0;
+    for (int k = 0; k < ncat; k++) {
+        double cum = 1.0;
+        if (k < ncat - 1) {
+            const double cut = orc_chi2_quantile((k + 1.0) / ncat, 2.0 * alpha) / (2.0 * alpha);
+            cum = orc_incgamma(cut * alpha, alpha + 1.0, lg1);
+        }
+        rates[k] = (cum - prev) * ncat;
+        prev = cum;
+    }
+    for (int k = 0; k < ncat; k++) sum += rates[k];
+    if (sum != (double)ncat)
+        for (int k = 0; k < ncat; k++) rates[k] *= ncat / sum;
+}
+
+/* Optimization::brent_opt (utils/optimization.cpp:174-310): 100 iterations, golden
+ * fraction 0.3819660, tolerance floor 1e-10, curvature of the last three points. */
+static double orc_brent(orc_fn1 f, void *ctx, double ax, double bx, double cx, double tol, double fa, double fb,
+                        double fc, double *fopt, double *f2opt) {
+    double lo = ax < cx ? ax : cx, hi = ax > cx ? ax : cx;
+    double x = bx, fx = fb, w, fw, v, fv;
+    if (fa < fc) { w = ax; fw = fa; v = cx; fv = fc; }
+    else { w = cx; fw = fc; v = ax; fv = fa; }
+    double d = 0.0, e = 0.0;
+    for (int it = 1; it <= 100; it++) {
+        const double xm = 0.5 * (lo + hi), tol1 = tol * fabs(x) + 1.0e-10, tol2 = 2.0 * tol1;
+        if (fabs(x - xm) <= tol2 - 0.5 * (hi - lo)) break;
+        int parabolic = 0;
+        if (fabs(e) > tol1) {
+            const double r = (x - w) * (fx - fv);
+            double q = (x - v) * (fx - fw), p = (x - v) * q - (x - w) * r;
+            q = 2.0 * (q - r);
+            if (q > 0.0) p = -p;
+            q = fabs(q);
+            const double etemp = e;
+            e = d;
+            if (!(fabs(p) >= fabs(0.5 * q * etemp) || p <= q * (lo - x) || p >= q * (hi - x))) {
+                parabolic = 1;
+                d = p / q;
+                const double u = x + d;
+                if (u - lo < tol2 || hi - u < tol2) d = (xm - x >= 0.0) ? fabs(tol1) : -fabs(tol1);
+            }
+        }
+        if (!parabolic) {
+            e = (x >= xm) ? lo - x : hi - x;
+            d = 0.3819660 * e;
+        }
+        const double u = fabs(d) >= tol1 ? x + d : x + (d >= 0.0 ? fabs(tol1) : -fabs(tol1));
+        const double fu = f(ctx, u);
+        if (fu <= fx) {
+            if (u >= x) lo = x; else hi = x;
+            v = w; fv = fw; w = x; fw = fx; x = u; fx = fu;
+        } else {
+            if (u < x) lo = u; else hi = u;
+            if (fu <= fw || w == x) { v = w; fv = fw; w = u; fw = fu; }
+            else if (fu <= fv || v == x || v == w) { v = u; fv = fu; }
+        }
+    }
+    *fopt = fx;
+    const double xw = x - w, wv = w - v, vx = v - x;
+    *f2opt = 2.0 * (fv * xw + fx * wv + fw * vx) / (v * v * xw + x * x * wv + w * w * vx);
+    return x;
+}
+
+/* Optimization::minimizeOneDimen (utils/optimization.cpp:322-372) */
+double orc_minimize_one_dimen(orc_fn1 f, void *ctx, double xmin, double xguess, double xmax, double tol, double *fx,
+                              double *f2x) {
+    if (xguess < xmin) xguess = xmin;
+    if (xguess > xmax) xguess = xmax;
+    const double eps = xguess * tol * 50.0;
+    double ax = xguess - eps, bx = xguess, cx = xguess + eps;
+    if (ax < xmin) ax = xmin;
+    if (cx > xmax) cx = xmax;
+    const double fb = f(ctx, bx);
+    double fa = ax < bx ? f(ctx, ax) : fb;
+    double fc = bx < cx ? f(ctx, cx) : fb;
+    if (fa < fb || fc < fb) {
+        if (ax != xmin) fa = f(ctx, xmin);
+        if (cx != xmax) fc = f(ctx, xmax);
+        ax = xmin;
+        cx = xmax;
+    }
+    const double x = orc_brent(f, ctx, ax, bx, cx, tol, fa, fb, fc, fx, f2x);
+    if (*fx > fb) {
+        *fx = f(ctx, bx);
+        return bx;
+    }
+    return x;
+}
+
+/* RateGamma::setGammaShape (model/rategamma.cpp:173-176) + clearAllPartialLH */
+void orc_set_gamma_shape(orc_tree *t, double alpha) {
+    t->gamma_shape = alpha;
+    orc_gamma_rates(alpha, t->ncat, t->rates);
+    orc_clear_all(t);
+}
+
+double orc_get_gamma_shape(const orc_tree *t) { return t->gamma_shape; }
+
+/* RateGamma::computeFunction (model/rategamma.cpp:178-185): -computeLikelihood() at the
+ * shape, the first leaf's branch */
+static double gamma_objective(void *ctx, double shape) {
+    orc_tree *t = (orc_tree *)ctx;
+    if (t->gamma_shape != shape) orc_set_gamma_shape(t, shape);
+    return -orc_branch_loglik(t, 0, t->nei[0], NULL);
+}
+
+/* RateGamma::optimizeParameters(gradient_epsilon, min_gamma, max_gamma)
+ * (model/rategamma.cpp:215-232), TOL_GAMMA_SHAPE = 0.001 (utils/tools.h:607) */
+double orc_optimize_gamma(orc_tree *t, double gradient_epsilon, double min_gamma, double max_gamma) {
+    double fx, f2x;
+    const double tol = gradient_epsilon > 0.001 ? gradient_epsilon : 0.001;
+    const double x = orc_minimize_one_dimen(gamma_objective, t, min_gamma, t->gamma_shape, max_gamma, tol, &fx, &f2x);
+    return -gamma_objective(t, x);
+}
+
+/* ModelFactory::optimizeParameters (model/modelfactory.cpp:1558-1720), BRLEN_OPTIMIZE,
+ * substitution model without free parameters (optimizeParametersOnly returns the rate
+ * model's logL, 1263-1315), Gamma rates (rescaleRates = 1, model/rateheterogeneity.h:288) */
+double orc_optimize_parameters(orc_tree *t, int num_iterations, double logl_epsilon, double gradient_epsilon,
+                               double min_gamma, double max_gamma, double min_bl, double max_bl, int max_nr) {
+    double cur = orc_branch_loglik(t, 0, t->nei[0], NULL);
+    for (int i = 2; i < num_iterations; i++) {
+        orc_optimize_all_branches(t, i < 3 ? i : 3, logl_epsilon, min_bl, max_bl, max_nr);
+        const double new_lh = orc_optimize_gamma(t, gradient_epsilon, min_gamma, max_gamma);
+        if (new_lh > cur + logl_epsilon) {
+            cur = new_lh;
+        } else {
+            cur = orc_optimize_all_branches(t, 100, logl_epsilon, min_bl, max_bl, max_nr);
+            break;
+        }
+    }
+    return cur;
 }

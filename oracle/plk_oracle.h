@@ -23,6 +23,12 @@
  *                                 model/modeldna.cpp:462-476, model/modelprotein.cpp:1240-1256
  *   RELL / UFBoot replicate update tree/iqtree.cpp:3852-3931
  *   NNI5 move evaluation          getBestNNIForBran tree/phylotree.cpp:4176-4429
+ *   discrete-Gamma rates          RateGamma::computeRatesMean model/rategamma.cpp:98-171
+ *   Brent one-dimensional search  Optimization::brent_opt / minimizeOneDimen
+ *                                 utils/optimization.cpp:174-372
+ *   Gamma shape optimisation      RateGamma::optimizeParameters model/rategamma.cpp:178-232
+ *   model-parameter loop          ModelFactory::optimizeParameters model/modelfactory.cpp:1263-1315,
+ *                                 1558-1720
  *
  * Memory layout follows the reference's pattern-interleaved SIMD layout with
  * ORC_V lanes: element (ptn, c, x) of a partial sits at double offset
@@ -40,6 +46,13 @@
  *                    rounding of pattern_lh); the update rule on a hand-worked sequence
  *   NNI              the move's score equals the logL of the swapped tree, built from
  *                    scratch, after the same five branch optimisations; the tree is restored
+ *   Gamma rates      scipy's exact category means (golden fixtures) to the algorithms'
+ *                    own accuracy; Yang (1994)'s published alpha = 0.5 table; mean 1
+ *   Brent            analytic functions against their known minima and scipy's bounded
+ *                    Brent; the bound and worse-than-start rules on constructed cases
+ *   shape / loop     the optimum agrees with a fine grid of oracle logL(alpha) and with
+ *                    scipy's bounded minimiser; the loop never lowers the logL and ends at
+ *                    a stationary point (branch df ~ 0 and alpha at the grid optimum)
  */
 #ifndef PLK_ORACLE_H
 #define PLK_ORACLE_H
@@ -100,6 +113,23 @@ void orc_rell_save_tree(int nsamples, int nptn, const float *counts, const doubl
                         double eps, const double *uniform, double *rell, double *boot_logl, int32_t *boot_counts,
                         double *boot_orig, int32_t *improved);
 int orc_state_unknown(int nstates, int seq_type);
+
+/* ---- model-parameter loop ---- */
+/* discrete-Gamma mean rates of shape alpha (RateGamma::computeRatesMean) */
+void orc_gamma_rates(double alpha, int ncat, double *rates);
+typedef double (*orc_fn1)(void *ctx, double x);
+/* Optimization::minimizeOneDimen + brent_opt on f(ctx, x); returns the abscissa, fx the
+ * minimum, f2x the curvature estimate */
+double orc_minimize_one_dimen(orc_fn1 f, void *ctx, double xmin, double xguess, double xmax, double tol, double *fx,
+                              double *f2x);
+/* set the Gamma shape (replaces the rates, all partials dirty); orc_set_model clears it */
+void orc_set_gamma_shape(orc_tree *t, double alpha);
+double orc_get_gamma_shape(const orc_tree *t);
+/* RateGamma::optimizeParameters: returns the logL, the shape is left in the tree */
+double orc_optimize_gamma(orc_tree *t, double gradient_epsilon, double min_gamma, double max_gamma);
+/* ModelFactory::optimizeParameters with branch lengths and the Gamma shape free */
+double orc_optimize_parameters(orc_tree *t, int num_iterations, double logl_epsilon, double gradient_epsilon,
+                               double min_gamma, double max_gamma, double min_bl, double max_bl, int max_nr);
 
 #ifdef __cplusplus
 }

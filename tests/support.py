@@ -29,6 +29,7 @@ def _cpu_has_avx512():
 
 
 _oracle = None
+ORC_FN1 = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double)
 
 
 def oracle_lib(lanes=None):
@@ -74,6 +75,15 @@ def oracle_lib(lanes=None):
         "orc_rell_save_tree": (None, [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float), dp, ctypes.c_double,
                                       ctypes.c_double, dp, dp, dp, ctypes.POINTER(ctypes.c_int32), dp,
                                       ctypes.POINTER(ctypes.c_int32)]),
+        "orc_gamma_rates": (None, [ctypes.c_double, ctypes.c_int, dp]),
+        "orc_minimize_one_dimen": (ctypes.c_double, [ORC_FN1, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                     ctypes.c_double, dp, dp]),
+        "orc_set_gamma_shape": (None, [P, ctypes.c_double]),
+        "orc_get_gamma_shape": (ctypes.c_double, [P]),
+        "orc_optimize_gamma": (ctypes.c_double, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+        "orc_optimize_parameters": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -159,6 +169,39 @@ class Oracle:
 
     def optimize_one(self, a, b, min_bl=1e-6, max_bl=10.0, max_nr=100):
         return self.lib.orc_optimize_one_branch(self.h, a, b, min_bl, max_bl, max_nr)
+
+    def set_gamma_shape(self, alpha):
+        self.lib.orc_set_gamma_shape(self.h, alpha)
+
+    def gamma_shape(self):
+        return self.lib.orc_get_gamma_shape(self.h)
+
+    def compute_likelihood(self):
+        return self.loglik(0, int(self.p.nei[0][0]))
+
+    def optimize_gamma(self, gradient_epsilon=1e-4, min_gamma=0.02, max_gamma=1000.0):
+        v = self.lib.orc_optimize_gamma(self.h, gradient_epsilon, min_gamma, max_gamma)
+        return self.gamma_shape(), v
+
+    def optimize_parameters(self, num_iterations=100, logl_epsilon=0.01, gradient_epsilon=1e-4, min_gamma=0.02,
+                            max_gamma=1000.0, min_bl=1e-6, max_bl=10.0, max_nr=100):
+        v = self.lib.orc_optimize_parameters(self.h, num_iterations, logl_epsilon, gradient_epsilon, min_gamma,
+                                             max_gamma, min_bl, max_bl, max_nr)
+        return self.gamma_shape(), v
+
+
+def minimize_one_dimen(fn, xmin, xguess, xmax, tol):
+    """The oracle's minimizeOneDimen on a Python function; returns (x, f(x), f2, calls)."""
+    calls = []
+
+    def cb(_ctx, x):
+        calls.append(x)
+        return float(fn(x))
+
+    c = ORC_FN1(cb)
+    fx, f2 = ctypes.c_double(), ctypes.c_double()
+    x = oracle_lib().orc_minimize_one_dimen(c, None, xmin, xguess, xmax, tol, ctypes.byref(fx), ctypes.byref(f2))
+    return x, fx.value, f2.value, calls
 
 
 # ---------------------------------------------------------------------------------------
