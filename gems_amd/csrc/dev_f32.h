@@ -20,9 +20,10 @@ namespace dev {
 
 // ---------------------------------------------------------------------------------------
 // P(t) in FP32 form: P_c[x][y] (FP64 math) written as MFMA A fragments
-// frag[c][xb][ks][lane] = P_c[xb*16 + lane%16][f32_kstate(ks, lane/16)] (zero padded), or plainly
-// [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][x] = (P_c tip_s)[x], rows
-// f32_trow(N) long (zero padded for the matrix-core forms).
+// frag[c][xb][ks][lane] = P_c[f32_orow(xb, lane%16)][f32_kstate(ks, lane/16)] (zero padded),
+// or plainly [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][r] =
+// (P_c tip_s)[f32_orow(r / 16, r % 16)], rows f32_trow(N) long (zero padded for the
+// matrix-core forms).
 template <int N>
 __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
     constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
@@ -47,19 +48,20 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
         } else {
             for (int idx = threadIdx.x; idx < FR; idx += blockDim.x) {
                 const int xb = idx / (KS * 64), ks = (idx / 64) % KS, l = idx % 64;
-                const int x = xb * 16 + (l & 15), y = f32_kstate(N, ks, l >> 4);
-                F[(int64_t)c * FR + idx] = (x < N && y < N) ? (float)Pm[x * N + y] : 0.0f;
+                const int x = f32_orow(N, xb, l & 15), y = f32_kstate(N, ks, l >> 4);
+                F[(int64_t)c * FR + idx] = (x >= 0 && y < N) ? (float)Pm[x * N + y] : 0.0f;
             }
         }
         if (j.leaf) {
             for (int idx = threadIdx.x; idx < su * TROW; idx += blockDim.x) {
-                const int s = idx / TROW, x = idx % TROW;
+                const int s = idx / TROW, r = idx % TROW, x = f32_orow(N, r / 16, r % 16);
                 double v = 0.0;
-                if (x < N)
+                if (x >= 0)
                     for (int y = 0; y < N; y++) v = fma(Pm[x * N + y], P.tipvec[s * N + y], v);
-                T[((int64_t)s * ncat + c) * TROW + x] = (float)v;
+                T[((int64_t)s * ncat + c) * TROW + r] = (float)v;
             }
-            for (int x = threadIdx.x; x < TROW; x += blockDim.x) T[((int64_t)su * ncat + c) * TROW + x] = x < N ? 1.0f : 0.0f;
+            for (int r = threadIdx.x; r < TROW; r += blockDim.x)
+                T[((int64_t)su * ncat + c) * TROW + r] = f32_orow(N, r / 16, r % 16) >= 0 ? 1.0f : 0.0f;
         }
         __syncthreads();
     }
@@ -103,6 +105,46 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     constexpr bool HOIST = N <= 32;
     constexpr int PBG = (N > 32) ? 1 : (NPB < 2 ? NPB : 2);
     constexpr int NB = HOIST ? NPB : PBG;  // pattern blocks whose child columns are held
+    // Carry (20 states): in a post-order op list nearly every op has the previous op's
+    // output as a child (the node's second-visited subtree, or its only internal child),
+    // and the host puts that child on the right.  In the row-group layout a lane's output
+    // rows 4g..4g+3 are exactly the child column entries it feeds to the next op's matrix
+    // instructions (k-steps 0-3), and state 16 + g comes from lane group 0 by shuffles; so
+    // the previous output is kept in registers, with its exponents, and that child is not
+    // read back from memory.  The values are the ones stored, so results are unchanged.
+    // The carried output is also written late: op k's stores are issued after op k+1's
+    // loads, so that op k+1 does not wait for them (a wavefront's vector memory operations
+    // complete in issue order, stores included) and they have a whole op to complete.
+    constexpr bool CARRY = f32_rg(N) && HOIST;
+    float cv[CARRY ? NPB : 1][KS];
+    int ce[CARRY ? NPB : 1];
+    int prev_out = -1;  // slot written by this tile's previous op (held in cv, ce)
+#pragma unroll
+    for (int j = 0; j < (CARRY ? NPB : 1); j++) {
+        ce[j] = 0;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) cv[j][ks] = 0.0f;
+    }
+    // Straight-line stores: a branch around them would make the compiler's wait for the
+    // next op's loads assume the path without stores, i.e. wait for these stores too.  So
+    // every lane stores, and where there is no real target (before the first op, waves
+    // without a category of their own, the exponents of all but the first wave, which keeps
+    // them) the address is the context's sink.
+    auto store_carried = [&]() {
+        if constexpr (CARRY) {
+            float *__restrict__ po = (prev_out >= 0 && active) ? part + (int64_t)prev_out * P.slot_stride + cbase : P.sink;
+            int *__restrict__ pe = (prev_out >= 0 && cw == 0)
+                                       ? expo + (int64_t)prev_out * P.scale_stride + (int64_t)tile * 64
+                                       : (int *)P.sink;
+#pragma unroll
+            for (int j = 0; j < NPB; j++) {
+                const int pl = (pb0 + j) * 16 + l16;
+                *(floatx4 *)(po + (lg * 64 + pl) * 4) = floatx4{cv[j][0], cv[j][1], cv[j][2], cv[j][3]};
+                po[(4 * 64 + pl) * 4 + lg] = cv[j][4];
+                pe[pl] = ce[j];
+            }
+        }
+    };
     int sp = 0;  // double-buffered s_max
     for (int k = 0; k < P.nops; k++) {
         const int ko = op_index(P, tile, k);
@@ -111,6 +153,14 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
         // code, so all loads of an op are issued together
         with_kind(op.kind, [&](auto KC) {
         constexpr int KIND = decltype(KC)::value;
+        const bool carry_r = CARRY && KIND != KIND_TT && op.right == prev_out;
+        if (CARRY && KIND == KIND_II && op.left == prev_out) {
+            // the previous output read from memory (an op list whose order does not put
+            // it on the right): write it first
+            store_carried();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
         const float *__restrict__ M = (const float *)P.mats + (int64_t)ko * P.mat_stride;
         const float *__restrict__ tabs = M + (int64_t)2 * ncat * FR;
         const float *__restrict__ R = part + (int64_t)op.right * P.slot_stride + cbase;
@@ -141,9 +191,11 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
         for (int j = 0; j < NPB; j++) {
             const int64_t pj = (int64_t)tile * 64 + (pb0 + j) * 16 + l16;
-            if (KIND == KIND_TI) ech[j] = expo[(int64_t)op.right * P.scale_stride + pj];
+            const int jc = CARRY ? j : 0;
+            if (KIND == KIND_TI) ech[j] = carry_r ? ce[jc] : expo[(int64_t)op.right * P.scale_stride + pj];
             else if (KIND == KIND_II)
-                ech[j] = expo[(int64_t)op.left * P.scale_stride + pj] + expo[(int64_t)op.right * P.scale_stride + pj];
+                ech[j] = expo[(int64_t)op.left * P.scale_stride + pj] +
+                         (carry_r ? ce[jc] : expo[(int64_t)op.right * P.scale_stride + pj]);
             else ech[j] = 0;
         }
         float bL[NB][KS], bR[NB][KS];
@@ -152,11 +204,17 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             const int col = pb * 16 + l16;
             if constexpr (f32_rg(N)) {
                 // row group lg (k-steps 0-3) as one 16-byte load, state 16 + lg (k-step 4)
+                const int jc = CARRY ? j : 0;
                 if (KIND != KIND_TT) {
-                    const floatx4 q = *(const floatx4 *)(R + (lg * 64 + col) * 4);
+                    if (carry_r) {
 #pragma unroll
-                    for (int ks = 0; ks < 4; ks++) bR[j][ks] = q[ks];
-                    bR[j][4] = R[(4 * 64 + col) * 4 + lg];
+                        for (int ks = 0; ks < KS; ks++) bR[j][ks] = cv[jc][ks];
+                    } else {
+                        const floatx4 q = *(const floatx4 *)(R + (lg * 64 + col) * 4);
+#pragma unroll
+                        for (int ks = 0; ks < 4; ks++) bR[j][ks] = q[ks];
+                        bR[j][4] = R[(4 * 64 + col) * 4 + lg];
+                    }
                 }
                 if (KIND == KIND_II) {
                     const floatx4 q = *(const floatx4 *)(L + (lg * 64 + col) * 4);
@@ -185,12 +243,25 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 const floatx4 *tr =
                     (const floatx4 *)(tabs + (int64_t)(P.su + 1) * ncat * TROW + ((int64_t)sr * ncat + c) * TROW + lg * 4);
 #pragma unroll
-                for (int xb = 0; xb < XB; xb++) tv[j][xb] = KIND == KIND_TT ? tl[xb * 4] * tr[xb * 4] : tl[xb * 4];
+                for (int xb = 0; xb < XB; xb++) {
+                    if (f32_rg(N) && xb == 1) {
+                        // block 1: only the lane group's first row (state 16 + lg) is non-zero
+                        const float a = ((const float *)tl)[16];
+                        tv[j][1] = floatx4{KIND == KIND_TT ? a * ((const float *)tr)[16] : a, 0.0f, 0.0f, 0.0f};
+                    } else {
+                        tv[j][xb] = KIND == KIND_TT ? tl[xb * 4] * tr[xb * 4] : tl[xb * 4];
+                    }
+                }
             }
         };
         if constexpr (HOIST) {
 #pragma unroll
             for (int j = 0; j < NPB; j++) load_block(j, pb0 + j);
+        }
+        if constexpr (CARRY) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            store_carried();
         }
 #pragma unroll
         for (int gi = 0; gi < NPB; gi += PBG) {
@@ -254,35 +325,41 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                 for (int cc = 0; cc < ncat; cc++) pm = fmaxf(pm, s_max[sp][cc][pl]);
                 const int e = norm_exponent(pm);
                 if constexpr (f32_rg(N)) {
-                    // rows 4lg..4lg+3 (row group lg) and, in lane group 0, rows 16-19
-                    floatx4 v0, v1;
+                    static_assert(!f32_rg(N) || CARRY, "the 20-state form always carries");
+                    // rows 4lg..4lg+3 (states 4lg..4lg+3) and state 16 + lg (first row of
+                    // block 1): exactly the next op's k-steps 0-3 and 4 in this lane;
+                    // store_carried writes them
+                    const int jc = gi + j;
 #pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        v0[r] = ldexpf(out[j][0][r], -e);
-                        v1[r] = ldexpf(out[j][1][r], -e);
-                    }
-                    if (active) *(floatx4 *)(outp + (lg * 64 + pl) * 4) = v0;
-                    if (active && lg == 0) *(floatx4 *)(outp + (4 * 64 + pl) * 4) = v1;
+                    for (int r = 0; r < 4; r++) cv[jc][r] = ldexpf(out[j][0][r], -e);
+                    cv[jc][4] = ldexpf(out[j][1][0], -e);
+                    ce[jc] = e + ech[gi + j];
                 } else {
 #pragma unroll
                     for (int xb = 0; xb < XB; xb++)
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            const int x = xb * 16 + lg * 4 + r;
-                            if (x < N && active) outp[x * 64 + pl] = ldexpf(out[j][xb][r], -e);
+                            const int x = f32_orow(N, xb, lg * 4 + r);
+                            if (x >= 0 && active) outp[x * 64 + pl] = ldexpf(out[j][xb][r], -e);
                         }
                 }
-                if (cw == 0 && lg == 0)  // one wave per pattern block keeps the exponents
+                if (!CARRY && cw == 0 && lg == 0)  // one wave per pattern block keeps the exponents
                     expo[(int64_t)op.out_slot * P.scale_stride + (int64_t)tile * 64 + pl] = e + ech[gi + j];
             }
             sp ^= 1;
         }
         });
+        prev_out = op.out_slot;
         // The next op reads these values (and exponents) in other lanes of this same
         // wavefront: a wavefront's vector memory operations are performed in order, and
         // the wavefront-scope fences keep the compiler from moving the next op's loads
         // above these stores.  No workgroup barrier, so a wave does not wait here for
         // its stores to complete; s_max is double-buffered across the per-op exchange.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (CARRY && prev_out >= 0) {
+        store_carried();
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
@@ -318,8 +395,8 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
         for (int xb = 0; xb < XB; xb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int x = xb * 16 + lg * 4 + r;
-                if (x < N) {
+                const int x = f32_orow(N, xb, lg * 4 + r);
+                if (x >= 0) {
                     const float a = atip ? (float)tv[x] : A[f32_xoff(N, x) + f32_loff(N, pb * 16 + l16)];
                     acc = fmaf(s_pi[x] * a, d[xb][r], acc);
                 }
@@ -443,8 +520,8 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
                 for (int xb = 0; xb < XB; xb++)
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
-                        const int x = xb * 16 + lg * 4 + r;
-                        if (x < N) outp[cbase(c) + f32_xoff(N, x)] = ldexpf(out[c][xb][r], -e);
+                        const int x = f32_orow(N, xb, lg * 4 + r);
+                        if (x >= 0) outp[cbase(c) + f32_xoff(N, x)] = ldexpf(out[c][xb][r], -e);
                     }
             }
             if (lg == 0) {
@@ -487,8 +564,8 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
         for (int xb = 0; xb < XB; xb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int x = xb * 16 + lg * 4 + r;
-                if (x < N) {
+                const int x = f32_orow(N, xb, lg * 4 + r);
+                if (x >= 0) {
                     const float a = atip ? (float)P.tipvec[sa * N + x] : A[f32_xoff(N, x)];
                     acc = fmaf((float)P.pi[x] * a, d[xb][r], acc);
                 }

@@ -115,6 +115,7 @@ KParams make_params(Ctx &c) {
     p.tile_sums = c.d_tile_sums;
     p.counter = c.d_counter;
     p.result = c.d_result;
+    p.sink = c.d_sink;
     return p;
 }
 
@@ -341,7 +342,9 @@ static int run_traversal(Ctx &c, KParams &p, int mode, bool used[3]) {
     if (small) plan_small(c, mode != MODE_PARTIALS, p.br_a, p.br_b, sp);
     if (small) c.jobs.clear();  // the level-parallel kernel forms P(t) itself
     std::vector<int32_t> perm;
-    const bool two_orders = !small && alt_order(c, c.pending, perm);
+    // (not for the 20-state FP32 traversal, which carries each op's output into the next
+    // op of the list's own order)
+    const bool two_orders = !small && !(f32 && c.n == 20) && alt_order(c, c.pending, perm);
     bool inline_ops = false;
     SmallInline inl;
     p.perm = nullptr;
@@ -532,6 +535,7 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMallocAsync((void **)&c->d_tile_sums, sizeof(double) * 2 * c->ntiles, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_result, sizeof(double) * 2, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_counter, sizeof(unsigned), c->stream));
+    CREATE_TRY(hipMallocAsync((void **)&c->d_sink, kSinkBytes, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_newton, sizeof(NewtonState), c->stream));
     const int n = nstates;
     const size_t model_doubles = n + 2 * n * n + 2 * ncat + 2 * (size_t)(c->su + 1) * n + n + n * n;
@@ -568,7 +572,7 @@ extern "C" int plk_destroy(plk_ctx *c) {
     void *bufs[] = {c->d_part, c->d_scale, c->d_states, c->d_freq, c->d_invar, c->d_mats, c->d_model,
                     c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
                     c->d_result, c->d_counter, c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres,
-                    c->d_mcount, c->d_newton, c->d_mnewton};
+                    c->d_mcount, c->d_newton, c->d_mnewton, c->d_sink};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, c->stream);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -688,6 +692,13 @@ extern "C" int plk_update_partials(plk_ctx *c, const plk_op *ops, int nops) {
             d.kind = KIND_II;
             d.left = o.child[0];
             d.right = o.child[1];
+            // the previous op's output goes right: the 20-state FP32 traversal keeps it in
+            // registers for that side (the two factors of the product commute exactly)
+            if (k > 0 && c->pending[k - 1].out_slot == o.child[0]) {
+                std::swap(d.left, d.right);
+                side_of0 = 1;
+                side_of1 = 0;
+            }
         }
         d.len_left = side_of0 == 0 ? o.len[0] : o.len[1];
         d.len_right = side_of0 == 0 ? o.len[1] : o.len[0];

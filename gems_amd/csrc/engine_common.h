@@ -30,6 +30,14 @@ __host__ __device__ constexpr int f32_loff(int n, int p) { return f32_rg(n) ? p 
 __host__ __device__ constexpr int f32_kstate(int n, int ks, int g) {
     return f32_rg(n) ? (ks < 4 ? 4 * g + ks : 16 + g) : ks * 4 + g;
 }
+// Matrix-core output row i (0-15) of x block xb holds state f32_orow(n, xb, i), or -1 for
+// a zero row.  20 states: block 1 holds states 16-19 in rows 0, 4, 8, 12, so lane group g
+// (output rows 4g..4g+3) has state 16 + g in its first row, which is the entry that lane
+// feeds to the next op's k-step 4.  Leaf-table rows follow the same order (row r of a
+// table = output row r % 16 of block r / 16).
+__host__ __device__ constexpr int f32_orow(int n, int xb, int i) {
+    return f32_rg(n) ? (xb == 0 ? i : ((i & 3) == 0 ? 16 + (i >> 2) : -1)) : (xb * 16 + i < n ? xb * 16 + i : -1);
+}
 
 namespace plk {
 
@@ -86,6 +94,7 @@ struct KParams {
     double *tile_sums;        // ntiles x 2
     unsigned *counter;
     double *result;           // 2
+    float *sink;              // kSinkBytes that stores of waves without a real target go to
     // level-parallel 4-state kernel (ops sorted by dependency level)
     const int32_t *lvl;       // nlevels + 1 op-index boundaries
     // a second valid order of the ops (nops indices) that half of the tiles follow, or
@@ -137,6 +146,7 @@ struct Ctx {
     void *d_theta = nullptr;
     double *d_buf_scale = nullptr, *d_ptn_lh = nullptr, *d_tile_sums = nullptr, *d_result = nullptr;
     unsigned *d_counter = nullptr;
+    float *d_sink = nullptr;
     NewtonState *d_newton = nullptr, *h_newton = nullptr;  // device state, pinned host mirror
     // host staging (pinned)
     void *h_stage = nullptr;
@@ -197,6 +207,11 @@ size_t small_stage_bytes(int ncat, int nops, int nlevels);
 
 // Shared host helpers (engine.cpp).
 KParams make_params(Ctx &c);
+// Write-only scratch of a context: a wave that has no real target for a store (a wave
+// without a category of its own, the exponent copies of all but one wave) writes there,
+// so that its stores need no branch.  Covers one 20-state tile-category in the FP32
+// row-group layout (5 x 64 x 4 floats).
+constexpr size_t kSinkBytes = 8192;
 bool valid_ref(const Ctx &c, int r);
 int ensure_op_capacity(Ctx &c, int nops);
 
