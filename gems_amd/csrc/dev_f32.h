@@ -102,7 +102,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     // category exchange): two for <= 32 states, whose loads for all of the wave's blocks
     // are issued at the op's start (HOIST); the 61-state model holds its 2 x 64 A
     // fragments in registers, so it loads and works on one block at a time
-    constexpr bool HOIST = N <= 32;
+    constexpr bool HOIST = N <= 32 && NPB <= 2;
     constexpr int PBG = (N > 32) ? 1 : (NPB < 2 ? NPB : 2);
     constexpr int NB = HOIST ? NPB : PBG;  // pattern blocks whose child columns are held
     // Carry (20 states): in a post-order op list nearly every op has the previous op's
@@ -115,7 +115,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     // The carried output is also written late: op k's stores are issued after op k+1's
     // loads, so that op k+1 does not wait for them (a wavefront's vector memory operations
     // complete in issue order, stores included) and they have a whole op to complete.
-    constexpr bool CARRY = f32_rg(N) && HOIST;
+    constexpr bool CARRY = f32_rg(N);
     float cv[CARRY ? NPB : 1][KS];
     int ce[CARRY ? NPB : 1];
     int prev_out = -1;  // slot written by this tile's previous op (held in cv, ce)
@@ -204,7 +204,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             const int col = pb * 16 + l16;
             if constexpr (f32_rg(N)) {
                 // row group lg (k-steps 0-3) as one 16-byte load, state 16 + lg (k-step 4)
-                const int jc = CARRY ? j : 0;
+                const int jc = CARRY ? pb - pb0 : 0;  // block of the wave (cv index)
                 if (KIND != KIND_TT) {
                     if (carry_r) {
 #pragma unroll
@@ -258,7 +258,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
             for (int j = 0; j < NPB; j++) load_block(j, pb0 + j);
         }
-        if constexpr (CARRY) {
+        if constexpr (CARRY && HOIST) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             store_carried();
@@ -269,6 +269,11 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
             if constexpr (!HOIST) {
 #pragma unroll
                 for (int j = 0; j < PBG; j++) load_block(j, g + j);
+                if (CARRY && gi == 0) {  // after the op's first loads
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    store_carried();
+                }
             }
             floatx4 out[PBG][XB];
             const int jb = HOIST ? gi : 0;  // first held block of this group
