@@ -91,7 +91,13 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     float *__restrict__ part = (float *)P.part;
     int *__restrict__ expo = (int *)P.scale;
     const int64_t cbase = ((int64_t)tile * ncat + c) * N * 64;  // (tile, c, x=0, lane=0)
-    __shared__ float s_max[2][PLK_GPU_MAX_CATS + 1][64];
+    // per-pattern maxima of the categories, [pattern][category] so two 16-byte LDS reads
+    // give a pattern's eight; entries of categories >= ncat stay zero, waves without a
+    // category of their own write s_spare
+    static_assert(PLK_GPU_MAX_CATS == 8, "two 16-byte reads of the category maxima");
+    __shared__ floatx4 s_max[2][64][2];
+    __shared__ float s_spare[64];
+    for (int i = threadIdx.x; i < 2 * 64 * 2; i += blockDim.x) s_max[i / 128][(i / 2) % 64][i % 2] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     __shared__ double s_cat[PLK_GPU_MAX_CATS][64];
     __shared__ float s_pi[64];
     __shared__ uint32_t s_tip[kStageTaxa * 16];
@@ -258,7 +264,15 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
             for (int j = 0; j < NPB; j++) load_block(j, pb0 + j);
         }
-        if constexpr (CARRY && HOIST) {
+        // 20-state waves with four blocks load a pair of blocks at a time: the first pair
+        // here, the next one as soon as the current pair's matrix instructions have read
+        // their operands (before its category exchange and stores)
+        constexpr bool EARLY = CARRY && !HOIST;
+        if constexpr (EARLY) {
+#pragma unroll
+            for (int j = 0; j < PBG; j++) load_block(j, pb0 + j);
+        }
+        if constexpr (CARRY) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             store_carried();
@@ -266,14 +280,9 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
         for (int gi = 0; gi < NPB; gi += PBG) {
             const int g = pb0 + gi;
-            if constexpr (!HOIST) {
+            if constexpr (!HOIST && !EARLY) {
 #pragma unroll
                 for (int j = 0; j < PBG; j++) load_block(j, g + j);
-                if (CARRY && gi == 0) {  // after the op's first loads
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    store_carried();
-                }
             }
             floatx4 out[PBG][XB];
             const int jb = HOIST ? gi : 0;  // first held block of this group
@@ -310,24 +319,34 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
                         for (int xb = 0; xb < XB; xb++) out[j][xb] = tv[jb + j][xb] * dR[xb];
                     }
                 }
-                // per-pattern maximum over this category (reduced over the 4 row groups)
+                // per-pattern maximum over this category (reduced over the 4 row groups);
+                // 20 states: rows 1-3 of block 1 are zero rows, left out
                 float m = 0.0f;
 #pragma unroll
                 for (int xb = 0; xb < XB; xb++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) m = fmaxf(m, fabsf(out[j][xb][r]));
+                    for (int r = 0; r < 4; r++)
+                        if (!(f32_rg(N) && xb == 1 && r > 0)) m = fmaxf(m, fabsf(out[j][xb][r]));
                 m = fmaxf(m, __shfl_xor(m, 16, 64));
                 m = fmaxf(m, __shfl_xor(m, 32, 64));
                 // all four row groups store the same value; inactive waves use a spare row
-                s_max[sp][active ? c : PLK_GPU_MAX_CATS][pb * 16 + l16] = m;
+                float *smx = active ? &((float *)&s_max[sp][pb * 16 + l16][0])[c] : &s_spare[pb * 16 + l16];
+                *smx = m;
+            }
+            if constexpr (EARLY) {
+                if (gi + PBG < NPB) {
+#pragma unroll
+                    for (int j = 0; j < PBG; j++) load_block(j, g + PBG + j);
+                }
             }
             __syncthreads();
             // maximum over all categories -> power-of-two renormalisation and exponent sum
 #pragma unroll
             for (int j = 0; j < PBG; j++) {
                 const int pl = (g + j) * 16 + l16;
-                float pm = 0.0f;
-                for (int cc = 0; cc < ncat; cc++) pm = fmaxf(pm, s_max[sp][cc][pl]);
+                const floatx4 c0 = s_max[sp][pl][0], c1 = s_max[sp][pl][1];
+                const float pm = fmaxf(fmaxf(fmaxf(c0[0], c0[1]), fmaxf(c0[2], c0[3])),
+                                       fmaxf(fmaxf(c1[0], c1[1]), fmaxf(c1[2], c1[3])));
                 const int e = norm_exponent(pm);
                 if constexpr (f32_rg(N)) {
                     static_assert(!f32_rg(N) || CARRY, "the 20-state form always carries");
