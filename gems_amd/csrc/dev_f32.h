@@ -115,9 +115,10 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     // output as a child (the node's second-visited subtree, or its only internal child),
     // and the host puts that child on the right.  In the row-group layout a lane's output
     // rows 4g..4g+3 are exactly the child column entries it feeds to the next op's matrix
-    // instructions (k-steps 0-3), and state 16 + g comes from lane group 0 by shuffles; so
-    // the previous output is kept in registers, with its exponents, and that child is not
-    // read back from memory.  The values are the ones stored, so results are unchanged.
+    // instructions (k-steps 0-3), and state 16 + g is the first row of its second x block
+    // (f32_orow), which feeds k-step 4; so the previous output is kept in registers, with
+    // its exponents, and that child is not read back from memory.  The values are the ones
+    // stored, so results are unchanged.
     // The carried output is also written late: op k's stores are issued after op k+1's
     // loads, so that op k+1 does not wait for them (a wavefront's vector memory operations
     // complete in issue order, stores included) and they have a whole op to complete.
@@ -318,6 +319,7 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
 #pragma unroll
                         for (int xb = 0; xb < XB; xb++) out[j][xb] = tv[jb + j][xb] * dR[xb];
                     }
+                    if constexpr (f32_rg(N)) out[j][1] = floatx4{out[j][1][0], 0.0f, 0.0f, 0.0f};  // zero rows
                 }
                 // per-pattern maximum over this category (reduced over the 4 row groups);
                 // 20 states: rows 1-3 of block 1 are zero rows, left out
