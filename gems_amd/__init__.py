@@ -103,6 +103,7 @@ _SIGS = {
     "plk_rate_matrix": (_int, [_int, _dp, _dp, _dp]),
     "plk_eigen": (_int, [_int, _dp, _dp, _dp, _dp, _dp]),
     "plk_gamma_rates": (_int, [_dbl, _int, _dp]),
+    "plk_gamma_rates_median": (_int, [_dbl, _int, _dp]),
     "plk_tip_lh": (_int, [_int, _int, _dp, _dp]),
     "plk_codon_exch": (_int, [_dbl, _dbl, _dp]),
     "plk_sim_create": (_int, [ctypes.POINTER(_P), _int, _int, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
@@ -137,6 +138,7 @@ _SIGS = {
     "plk_tree_queue_branch": (_int, [_P, _int, _int, _int, ctypes.POINTER(_int), ctypes.POINTER(_int), _dp]),
     "plk_tree_set_gamma_shape": (_int, [_P, _dbl]),
     "plk_tree_get_gamma_shape": (_int, [_P, _dp]),
+    "plk_tree_set_gamma_median": (_int, [_P, _int]),
     "plk_tree_optimize_gamma": (_int, [_P, _dbl, _dbl, _dbl, _dp, _dp]),
     "plk_tree_optimize_parameters": (_int, [_P, _int, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _dp, _dp]),
     "plk_sim_create_on_tree": (_int, [ctypes.POINTER(_P), _int, _int, _ip, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
@@ -357,6 +359,9 @@ class Tree:
 
     def set_gamma_shape(self, alpha):
         _check(lib().plk_tree_set_gamma_shape(self.h, alpha))
+
+    def set_gamma_median(self, median):
+        _check(lib().plk_tree_set_gamma_median(self.h, int(median)))
 
     def gamma_shape(self):
         v = ctypes.c_double()

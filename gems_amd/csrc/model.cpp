@@ -254,6 +254,23 @@ extern "C" int plk_gamma_rates(double alpha, int ncat, double *rates) {
     return PLK_OK;
 }
 
+extern "C" int plk_gamma_rates_median(double alpha, int ncat, double *rates) {
+    if (!(alpha > 0.0) || ncat < 1 || ncat > PLK_MAX_CATS || !rates)
+        return plk::fail(PLK_ERR_ARG, "plk_gamma_rates_median: bad arguments");
+    if (ncat == 1) {
+        rates[0] = 1.0;
+        return PLK_OK;
+    }
+    double sum = 0.0;
+    for (int k = 0; k < ncat; k++) {
+        const double q = point_chi2((2.0 * k + 1.0) / (2.0 * ncat), 2.0 * alpha) / (2.0 * alpha);
+        rates[k] = std::fabs(q);
+        sum += rates[k];
+    }
+    for (int k = 0; k < ncat; k++) rates[k] = rates[k] * ncat / sum;
+    return PLK_OK;
+}
+
 extern "C" int plk_tip_lh(int n, int seq_type, const double *inv_evec, double *tip_lh) {
     if (n < 2 || n > PLK_MAX_STATES || !inv_evec || !tip_lh) return plk::fail(PLK_ERR_ARG, "plk_tip_lh: bad arguments");
     int su = plk_state_unknown(n, seq_type);

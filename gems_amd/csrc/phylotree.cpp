@@ -38,6 +38,7 @@ class PhyloTreeGPU {
     std::vector<double> m_eval, m_evec, m_inv, m_rates, m_props, m_tip;
     bool model_stored = false;
     double gamma_shape = 0.0;
+    bool gamma_median = false;  // category rates at the medians (RateGamma's cut_median)
 
     bool is_leaf(int v) const { return v < ntaxa; }
 
@@ -554,7 +555,7 @@ class PhyloTreeGPU {
     int set_gamma(double alpha) {
         if (!model_stored) return fail(PLK_ERR_STATE, "Gamma shape needs a model (plk_tree_set_model) first");
         std::vector<double> r(ncat);
-        int rc = plk_gamma_rates(alpha, ncat, r.data());
+        int rc = gamma_median ? plk_gamma_rates_median(alpha, ncat, r.data()) : plk_gamma_rates(alpha, ncat, r.data());
         if (rc) return rc;
         m_rates = r;
         gamma_shape = alpha;
@@ -687,6 +688,12 @@ extern "C" int plk_tree_set_model(plk_tree *t, const double *eval, const double 
 extern "C" int plk_tree_set_gamma_shape(plk_tree *t, double alpha) {
     if (!t || !(alpha > 0.0) || !std::isfinite(alpha)) return fail(PLK_ERR_ARG, "plk_tree_set_gamma_shape: bad arguments");
     return t->set_gamma(alpha);
+}
+
+extern "C" int plk_tree_set_gamma_median(plk_tree *t, int median) {
+    if (!t) return fail(PLK_ERR_ARG, "plk_tree_set_gamma_median: null tree");
+    t->gamma_median = median != 0;
+    return PLK_OK;
 }
 
 extern "C" int plk_tree_get_gamma_shape(plk_tree *t, double *alpha) {

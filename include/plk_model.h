@@ -39,8 +39,13 @@ int plk_rate_matrix(int n, const double *exch, const double *pi, double *Q);
 int plk_eigen(int n, const double *Q, const double *pi, double *eval, double *evec,
               double *inv_evec);
 
-/* Mean-of-category discrete Gamma rates (Yang 1994), rescaled to mean 1. */
+/* Mean-of-category discrete Gamma rates (Yang 1994), rescaled to mean 1
+ * (RateGamma::computeRatesMean, model/rategamma.cpp:158-171). */
 int plk_gamma_rates(double alpha, int ncat, double *rates);
+/* Median-of-category rates (the cut_median branch of RateGamma::computeRates,
+ * model/rategamma.cpp:111-126): the Gamma(alpha, 1/alpha) quantile at (2k+1)/(2 ncat) for
+ * category k, rescaled to mean 1. */
+int plk_gamma_rates_median(double alpha, int ncat, double *rates);
 
 /* tip_partial_lh[(su+1) x n] = Uinv * tip likelihood of each state */
 int plk_tip_lh(int n, int seq_type, const double *inv_evec, double *tip_lh);

@@ -85,7 +85,8 @@ int plk_tree_all_branch_derivs(plk_tree *t, int32_t *branch_a, int32_t *branch_b
 int plk_tree_counters(plk_tree *t, int64_t *out3);
 
 /* ---- model-parameter loop ------------------------------------------------------------
- * The rate model is discrete Gamma with ncat mean-rate categories; the substitution model
+ * The rate model is discrete Gamma with ncat mean-rate categories (median-rate with
+ * plk_tree_set_gamma_median); the substitution model
  * (eigen system, frequencies) stays as plk_tree_set_model gave it.  The tree keeps a copy
  * of that model and re-uploads it with the new category rates whenever the shape changes
  * (all partials become dirty).  logL values are computeLikelihood()'s (leaf 0's branch). */
@@ -97,6 +98,10 @@ int plk_tree_counters(plk_tree *t, int64_t *out3);
 int plk_tree_set_gamma_shape(plk_tree *t, double alpha);
 /* current shape, 0 when none is set */
 int plk_tree_get_gamma_shape(plk_tree *t, double *alpha);
+/* median != 0: later shapes (plk_tree_set_gamma_shape, the searches) take their rates at
+ * the category medians (plk_gamma_rates_median; RateGamma's cut_median,
+ * model/rategamma.cpp:111-126) instead of the means */
+int plk_tree_set_gamma_median(plk_tree *t, int median);
 /* RateGamma::optimizeParameters(gradient_epsilon, min_gamma, max_gamma)
  * (model/rategamma.cpp:215-232): Brent's method behind Optimization::minimizeOneDimen
  * (utils/optimization.cpp:174-372) over the shape, starting at the current one, with

@@ -52,6 +52,7 @@ struct orc_tree {
     size_t mats_cap;
     long long updates;
     double gamma_shape;        /* 0: rates are the caller's own */
+    int gamma_median;          /* category rates at the medians (cut_median) */
 };
 
 static void *xalloc(size_t bytes) {
@@ -1114,6 +1115,17 @@ void orc_gamma_rates(double alpha, int ncat, double *rates) {
         for (int k = 0; k < ncat; k++) rates[k] *= ncat / sum;
 }
 
+/* the cut_median branch of RateGamma::computeRates (model/rategamma.cpp:111-126) */
+void orc_gamma_rates_median(double alpha, int ncat, double *rates) {
+    if (ncat == 1) { rates[0] = 1.0; return; }
+    double sum = 0.0;
+    for (int k = 0; k < ncat; k++) {
+        rates[k] = fabs(orc_chi2_quantile((2.0 * k + 1.0) / (2.0 * ncat), 2.0 * alpha) / (2.0 * alpha));
+        sum += rates[k];
+    }
+    for (int k = 0; k < ncat; k++) rates[k] = rates[k] * ncat / sum;
+}
+
 /* Optimization::brent_opt (utils/optimization.cpp:174-310): 100 iterations, golden
  * fraction 0.3819660, tolerance floor 1e-10, curvature of the last three points. */
 static double orc_brent(orc_fn1 f, void *ctx, double ax, double bx, double cx, double tol, double fa, double fb,
@@ -1192,9 +1204,12 @@ double orc_minimize_one_dimen(orc_fn1 f, void *ctx, double xmin, double xguess, 
 /* RateGamma::setGammaShape (model/rategamma.cpp:173-176) + clearAllPartialLH */
 void orc_set_gamma_shape(orc_tree *t, double alpha) {
     t->gamma_shape = alpha;
-    orc_gamma_rates(alpha, t->ncat, t->rates);
+    if (t->gamma_median) orc_gamma_rates_median(alpha, t->ncat, t->rates);
+    else orc_gamma_rates(alpha, t->ncat, t->rates);
     orc_clear_all(t);
 }
+
+void orc_set_gamma_median(orc_tree *t, int median) { t->gamma_median = median != 0; }
 
 double orc_get_gamma_shape(const orc_tree *t) { return t->gamma_shape; }
 

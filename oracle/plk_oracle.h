@@ -117,6 +117,10 @@ int orc_state_unknown(int nstates, int seq_type);
 /* ---- model-parameter loop ---- */
 /* discrete-Gamma mean rates of shape alpha (RateGamma::computeRatesMean) */
 void orc_gamma_rates(double alpha, int ncat, double *rates);
+/* median-of-category rates (RateGamma::computeRates with cut_median) */
+void orc_gamma_rates_median(double alpha, int ncat, double *rates);
+/* rates of later shapes at the category medians instead of the means */
+void orc_set_gamma_median(orc_tree *t, int median);
 typedef double (*orc_fn1)(void *ctx, double x);
 /* Optimization::minimizeOneDimen + brent_opt on f(ctx, x); returns the abscissa, fx the
  * minimum, f2x the curvature estimate */

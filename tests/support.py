@@ -76,6 +76,8 @@ def oracle_lib(lanes=None):
                                       ctypes.c_double, dp, dp, dp, ctypes.POINTER(ctypes.c_int32), dp,
                                       ctypes.POINTER(ctypes.c_int32)]),
         "orc_gamma_rates": (None, [ctypes.c_double, ctypes.c_int, dp]),
+        "orc_gamma_rates_median": (None, [ctypes.c_double, ctypes.c_int, dp]),
+        "orc_set_gamma_median": (None, [P, ctypes.c_int]),
         "orc_minimize_one_dimen": (ctypes.c_double, [ORC_FN1, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                      ctypes.c_double, dp, dp]),
         "orc_set_gamma_shape": (None, [P, ctypes.c_double]),
@@ -175,6 +177,9 @@ class Oracle:
 
     def gamma_shape(self):
         return self.lib.orc_get_gamma_shape(self.h)
+
+    def set_gamma_median(self, median):
+        self.lib.orc_set_gamma_median(self.h, int(median))
 
     def compute_likelihood(self):
         return self.loglik(0, int(self.p.nei[0][0]))

@@ -118,6 +118,39 @@ def test_optimize_parameters_matches_oracle(name, seq, T, sites, prec, safe):
     o.close()
 
 
+@pytest.mark.parametrize("prec", [G.FP64, G.FP32], ids=["f64", "f32"])
+def test_optimize_gamma_median_rates(prec):
+    """The cut_median rate model through the same search: logL(alpha) and the search
+    against the oracle's median-rate restatement."""
+    p = G.Problem(G.SEQ_DNA, 16, 4000, mean_len=0.15, alpha=0.7, seed=41)
+    say("optimize_gamma median", prec, "nptn", p.nptn)
+    g = G.Tree(p, precision=prec)
+    o = S.Oracle(p, nthreads=4)
+    g.set_gamma_median(True)
+    o.set_gamma_median(True)
+    g.set_gamma_shape(1.0)
+    o.set_gamma_shape(1.0)
+    assert rel(g.compute_likelihood(), o.compute_likelihood()) <= logl_tol(prec)
+    ag, lg = g.optimize_gamma(gradient_epsilon=1e-4)
+    ao, lo = o.optimize_gamma(gradient_epsilon=1e-4)
+    say(f"  alpha gpu {ag:.10f} oracle {ao:.10f}  logL gpu {lg:.10f} oracle {lo:.10f}")
+
+    def f_dev(x):
+        g.set_gamma_shape(x)
+        return -g.compute_likelihood()
+
+    xb, _fb, _f2, _calls = S.minimize_one_dimen(f_dev, 0.02, 1.0, 1000.0, 1e-3)
+    assert rel(xb, ag) <= 1e-12
+    assert rel(ag, ao) <= ALPHA_END and abs(lg - lo) <= 1e-3
+    # a different rate model: the mean-rate optimum differs
+    g.set_gamma_median(False)
+    g.set_gamma_shape(1.0)
+    am, _ = g.optimize_gamma(gradient_epsilon=1e-4)
+    assert abs(am - ag) > 1e-3 * ag
+    g.close()
+    o.close()
+
+
 def test_gamma_needs_a_shape_and_model():
     p = G.Problem(G.SEQ_DNA, 6, 300, seed=3)
     g = G.Tree(p)

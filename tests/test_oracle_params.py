@@ -9,6 +9,7 @@ import os
 import numpy as np
 import pytest
 import scipy.optimize
+import scipy.stats
 
 import gems_amd as G
 import support as S
@@ -44,6 +45,21 @@ def test_gamma_rates_same_as_library_host(alpha, k):
     # the same published algorithms, written apart: rounding-level differences, which the
     # cut-point iterations amplify to ~1e-11 at the largest shapes
     np.testing.assert_allclose(orc_rates(alpha, k), r, rtol=1e-10, atol=1e-300)
+
+
+@pytest.mark.parametrize("alpha", [0.05, 0.2, 0.5, 1.0, 3.0, 20.0, 200.0])
+@pytest.mark.parametrize("k", [2, 4, 6, 8])
+def test_median_rates_vs_scipy_and_library(alpha, k):
+    """cut_median rates: Gamma(alpha, 1/alpha) quantiles at (2i+1)/(2k), mean 1; the
+    published chi-square routine is accurate to ~1e-6 relative."""
+    o = np.zeros(k)
+    S.oracle_lib().orc_gamma_rates_median(alpha, k, S._dp(o))
+    q = scipy.stats.gamma.ppf((2 * np.arange(k) + 1) / (2.0 * k), a=alpha, scale=1.0 / alpha)
+    np.testing.assert_allclose(o, q * k / q.sum(), rtol=2e-5)
+    assert abs(o.mean() - 1.0) < 1e-12 and np.all(np.diff(o) > 0)
+    r = np.zeros(k)
+    assert G.lib().plk_gamma_rates_median(alpha, k, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    np.testing.assert_allclose(o, r, rtol=1e-10)
 
 
 @pytest.mark.parametrize("fn,xmin,guess,xmax,xstar", [
