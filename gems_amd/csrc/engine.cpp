@@ -475,8 +475,6 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     if (ntaxa < 3 || nptn < 1 || nslots < 1) return fail(PLK_ERR_ARG, "plk_create: need ntaxa >= 3, nptn >= 1, nslots >= 1");
     if (precision != PLK_FP64 && precision != PLK_FP32) return fail(PLK_ERR_ARG, "plk_create: bad precision form");
     if (scaling != PLK_SCALE_NORM && scaling != PLK_SCALE_SAFE) return fail(PLK_ERR_ARG, "plk_create: bad scaling");
-    if (precision == PLK_FP32 && nstates == 61 && ncat > 4)
-        return fail(PLK_ERR_ARG, "plk_create: the FP32 form of the 61-state model supports ncat <= 4");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(PLK_ERR_HIP, "plk_create: no HIP device visible (the engine has no CPU path)");

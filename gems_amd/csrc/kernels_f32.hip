@@ -10,15 +10,19 @@ __global__ void __launch_bounds__(256) k_pmatrix_f32(KParams P, const PJob *__re
     pmatrix_f32_body<N>(P, jobs[blockIdx.x]);
 }
 
-// WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1; WIDE
-// allows 5-8 categories or WPC > 1.  The 20-state forms keep four waves per SIMD resident
-// (128 registers): at three, a 49k-pattern alignment's 770 tiles would overflow the 768
-// workgroup slots of one round; the 61-state form may use the whole register file.
+// WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1; WIDE = 1
+// allows up to 16 waves (5-8 categories or WPC > 1, 20 states), WIDE = 2 up to 8 (61
+// states with 5-8 categories: two waves per SIMD, so half the register file each and
+// spills; the 61-state A fragments alone fill 128 registers).  The 20-state forms keep
+// four waves per SIMD resident (128 registers): at three, a 49k-pattern alignment's 770
+// tiles would overflow the 768 workgroup slots of one round.
 template <int N, int WIDE>
 constexpr int mfma_min_waves() { return (N <= 32 && !WIDE) ? 4 : 1; }
+template <int WIDE>
+constexpr int mfma_threads() { return WIDE == 1 ? 64 * 16 : WIDE == 2 ? 64 * 8 : 256; }
 
 template <int N, int MODE, int WIDE, int WPC>
-__global__ void __launch_bounds__(WIDE ? 64 * 4 * 4 : 256, (mfma_min_waves<N, WIDE>())) k_traverse_f32_mfma(KParams P) {
+__global__ void __launch_bounds__(mfma_threads<WIDE>(), (mfma_min_waves<N, WIDE>())) k_traverse_f32_mfma(KParams P) {
     traverse_f32_mfma_body<N, MODE, WPC>(P, blockIdx.x);
 }
 
@@ -54,7 +58,19 @@ hipError_t traverse_f32_n(const KParams &p, int mode, hipStream_t s) {
             return hipGetLastError();
         }
         int wpc = p.wpc;
-        if (N > 32 || wpc < 1 || p.ncat * wpc > 16) wpc = 1;
+        if (N > 32) {
+            // 61 states: one wave per category (a second one would not fit beside it)
+            dim3 block(64 * p.ncat);
+            if (p.ncat <= 4) {
+                if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 0, 1>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 0, 1>), grid, block, 0, s, p);
+            } else {
+                if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 2, 1>), grid, block, 0, s, p);
+                else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 2, 1>), grid, block, 0, s, p);
+            }
+            return hipGetLastError();
+        }
+        if (wpc < 1 || p.ncat * wpc > 16) wpc = 1;
         dim3 block(64 * p.ncat * wpc);
         if (wpc == 1 && p.ncat <= 4) {
             if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 0, 1>), grid, block, 0, s, p);

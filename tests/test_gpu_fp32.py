@@ -148,3 +148,17 @@ def test_fp32_tile_shapes_identical():
     a = gc.full_traversal()
     gc.set_tile_shape(0)
     assert gc.full_traversal() == a
+
+
+@pytest.mark.parametrize("seq,ncat", [(G.SEQ_CODON, 6), (G.SEQ_PROTEIN, 8), (G.SEQ_DNA, 8)], ids=["codon6", "protein8", "dna8"])
+def test_fp32_many_categories(seq, ncat):
+    """Five to eight rate categories (the wide workgroup shapes) against the oracle."""
+    say("fp32 many categories", seq, ncat)
+    p = G.Problem(seq, 10, 400, mean_len=0.15, alpha=0.6, ncat=ncat, seed=29)
+    g = G.Tree(p, precision=G.FP32)
+    o = S.Oracle(p, safe=(seq == G.SEQ_CODON), nthreads=4)
+    for (u, v) in branches(p, 3) + [(0, int(p.nei[0, 0]))]:
+        lg, lo = g.loglik(u, v), o.loglik(u, v)
+        assert rel(lg, lo) <= LOGL_TOL, (u, v, lg, lo)
+    g.close()
+    o.close()
