@@ -76,16 +76,27 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
 // (WPC = 1: one wave does all 64 patterns and reuses its A fragments four times;
 // WPC = 2 or 4: more waves in flight per tile).  Wave w handles category w / WPC and
 // pattern blocks [(w % WPC) * 4/WPC, ...).
+//
+// Single-block units (blk >= 0, WPC = 4): the workgroup has one wave per category and
+// covers only pattern block blk of the tile; `unit` / `nunits` index the workgroup's
+// logL partial sum (by default the tile / the tile count).  A launch whose tile count is
+// just above a multiple of the compute-unit count runs its last tiles as such units, so
+// no compute unit carries a whole extra tile.
 template <int N, int MODE, int WPC = 1>
-__device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) {
+__device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile, const int blk = -1, int unit = -1,
+                                              int nunits = -1) {
     constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
     constexpr int NPB = 4 / WPC;  // pattern blocks per wave
     const int lane = threadIdx.x & 63, l16 = lane & 15, lg = lane >> 4;
     const int ncat = P.ncat;
+    if (unit < 0) {
+        unit = tile;
+        nunits = P.ntiles;
+    }
     // waves beyond ncat * WPC (batched launches) only join barriers
     const int wv = wave_id();
-    const int cw = wv / WPC;
-    const int pb0 = (wv % WPC) * NPB;
+    const int cw = blk >= 0 ? wv : wv / WPC;
+    const int pb0 = blk >= 0 ? blk : (wv % WPC) * NPB;
     const bool active = cw < ncat;
     const int c = active ? cw : 0;
     float *__restrict__ part = (float *)P.part;
@@ -435,15 +446,16 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile) 
     double r0 = 0.0;
     if (wv == 0) {
         const int64_t p = (int64_t)tile * 64 + lane;
+        const bool mine = p < P.nptn && (blk < 0 || (lane >> 4) == blk);  // this workgroup's patterns
         double lh = 0.0;
         for (int cc = 0; cc < ncat; cc++) lh += P.props[cc] * s_cat[cc][lane];
         lh = fabs(lh) + P.invar[p];
         const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
         const double l = log(lh) + e * kLn2;
-        if (p < P.nptn) P.ptn_lh[p] = l;
-        r0 = wave_sum(p < P.nptn ? l * P.freq[p] : 0.0);
+        if (mine) P.ptn_lh[p] = l;
+        r0 = wave_sum(mine ? l * P.freq[p] : 0.0);
     }
-    finish_tiles(P, r0, 0.0, tile, P.ntiles);
+    finish_tiles(P, r0, 0.0, unit, nunits);
 }
 
 // ---------------------------------------------------------------------------------------

@@ -116,6 +116,8 @@ KParams make_params(Ctx &c) {
     p.counter = c.d_counter;
     p.result = c.d_result;
     p.sink = c.d_sink;
+    p.ncu = c.ncu;
+    p.split_full = 0;
     return p;
 }
 
@@ -493,6 +495,7 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
         });
     }
     c->dev = dev;
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) c->ncu = 0;
     c->n = nstates;
     c->ncat = ncat;
     c->block = nstates * ncat;
@@ -530,7 +533,8 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMallocAsync((void **)&c->d_theta, sizeof(double) * slot, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_buf_scale, sizeof(double) * c->nptn_pad, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_ptn_lh, sizeof(double) * c->nptn_pad, c->stream));
-    CREATE_TRY(hipMallocAsync((void **)&c->d_tile_sums, sizeof(double) * 2 * c->ntiles, c->stream));
+    // per-tile partial sums; a split launch (single-block units) has up to 4 per tile
+    CREATE_TRY(hipMallocAsync((void **)&c->d_tile_sums, sizeof(double) * 2 * 4 * c->ntiles, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_result, sizeof(double) * 2, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_counter, sizeof(unsigned), c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_sink, kSinkBytes, c->stream));

@@ -95,6 +95,8 @@ struct KParams {
     unsigned *counter;
     double *result;           // 2
     float *sink;              // kSinkBytes that stores of waves without a real target go to
+    int ncu;                  // compute units of the device
+    int split_full;           // split launches: tiles handled whole (the rest as single blocks)
     // level-parallel 4-state kernel (ops sorted by dependency level)
     const int32_t *lvl;       // nlevels + 1 op-index boundaries
     // a second valid order of the ops (nops indices) that half of the tiles follow, or
@@ -147,6 +149,7 @@ struct Ctx {
     double *d_buf_scale = nullptr, *d_ptn_lh = nullptr, *d_tile_sums = nullptr, *d_result = nullptr;
     unsigned *d_counter = nullptr;
     float *d_sink = nullptr;
+    int ncu = 0;  // compute units of the device
     NewtonState *d_newton = nullptr, *h_newton = nullptr;  // device state, pinned host mirror
     // host staging (pinned)
     void *h_stage = nullptr;

@@ -26,6 +26,19 @@ __global__ void __launch_bounds__(mfma_threads<WIDE>(), (mfma_min_waves<N, WIDE>
     traverse_f32_mfma_body<N, MODE, WPC>(P, blockIdx.x);
 }
 
+// Whole tiles for blockIdx < P.split_full, then single 16-pattern blocks of the remaining
+// tiles (four units per tile), all with one wave per category.
+template <int N, int MODE>
+__global__ void __launch_bounds__(256, 4) k_traverse_f32_mfma_split(KParams P) {
+    const int b = blockIdx.x, full = P.split_full;
+    if (b < full) {
+        traverse_f32_mfma_body<N, MODE, 1>(P, b, -1, b, gridDim.x);
+    } else {
+        const int u = b - full;
+        traverse_f32_mfma_body<N, MODE, 4>(P, full + u / 4, u % 4, b, gridDim.x);
+    }
+}
+
 template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_traverse_f32_pbw(KParams P) {
     traverse_f32_pbw_body<N, MODE>(P, blockIdx.x);
@@ -72,6 +85,22 @@ hipError_t traverse_f32_n(const KParams &p, int mode, hipStream_t s) {
         }
         if (wpc < 1 || p.ncat * wpc > 16) wpc = 1;
         dim3 block(64 * p.ncat * wpc);
+        if constexpr (N <= 32) {
+        if (wpc == 1 && p.ncat <= 4 && p.ncu > 0) {
+            // tiles beyond a multiple of the compute-unit count (up to a quarter of the
+            // compute units) run as single-block units, so that every compute unit holds
+            // the same number of whole tiles (four workgroups of this shape fit one)
+            const int per = p.ntiles / p.ncu, extra = p.ntiles - per * p.ncu;
+            if (per >= 1 && per <= 3 && extra > 0 && 4 * extra <= p.ncu) {
+                KParams q = p;
+                q.split_full = p.ntiles - extra;
+                dim3 g2(q.split_full + 4 * extra);
+                if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma_split<N, MODE_PARTIALS>), g2, block, 0, s, q);
+                else hipLaunchKernelGGL((k_traverse_f32_mfma_split<N, MODE_LOGL>), g2, block, 0, s, q);
+                return hipGetLastError();
+            }
+        }
+        }
         if (wpc == 1 && p.ncat <= 4) {
             if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_PARTIALS, 0, 1>), grid, block, 0, s, p);
             else hipLaunchKernelGGL((k_traverse_f32_mfma<N, MODE_LOGL, 0, 1>), grid, block, 0, s, p);

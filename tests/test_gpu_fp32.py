@@ -162,3 +162,26 @@ def test_fp32_many_categories(seq, ncat):
         assert rel(lg, lo) <= LOGL_TOL, (u, v, lg, lo)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("sites", [16600, 16640], ids=["257tiles", "258tiles"])
+def test_fp32_split_units(sites):
+    """Tile counts just above the 256 compute units: the last tiles run as single-block
+    units.  Per-pattern logL is the same as with whole tiles (another tile shape), the total
+    differs only by summation order, and both match the oracle."""
+    p = G.Problem(G.SEQ_PROTEIN, 60, sites, mean_len=1.0, seed=5)
+    say("fp32 split units", sites, "nptn", p.nptn, "tiles", (p.nptn + 63) // 64)
+    o = S.Oracle(p, nthreads=4)
+    a, b = 0, int(p.nei[0, 0])
+    lo, plo = o.loglik(a, b, with_patterns=True)
+    res = {}
+    for shape in (1, 2):
+        g = G.Tree(p, precision=G.FP32)
+        g.set_tile_shape(shape)
+        res[shape] = g.loglik(a, b, with_patterns=True)
+        g.close()
+    (l1, p1), (l2, p2) = res[1], res[2]
+    assert np.array_equal(p1, p2)
+    assert rel(l1, l2) <= 1e-12
+    assert rel(l1, lo) <= LOGL_TOL and np.max(np.abs(p1 - plo)) < 1e-3
+    o.close()
