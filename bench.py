@@ -126,6 +126,18 @@ def main():
         return {"ups": ups, "ms_per_step": el / args.steps * 1e3, "kernel_avg_ms": k_avg_s * 1e3,
                 "achieved": achieved, "bytes_per_update": bpu, "logl": logl, "tile_waves_per_category": tile}
 
+    def f32_kernel_name(tile):
+        """The kernel the FP32 launch of this problem runs (the rule of traverse_f32_n)."""
+        if tile == 0:
+            return "k_traverse_f32_pbw<20,MODE_LOGL>"
+        ntiles, ncu = (p.nptn + 63) // 64, G.device_units()
+        if tile == 1 and ncu > 0:
+            per, extra = ntiles // ncu, ntiles % ncu
+            if 1 <= per <= 3 and 0 < extra and 4 * extra <= ncu:
+                return (f"k_traverse_f32_mfma_split<20,MODE_LOGL> ({ntiles - extra} whole tiles + "
+                        f"{4 * extra} single-block units on {ncu} compute units)")
+        return f"k_traverse_f32_mfma<20,MODE_LOGL,wpc={tile}>"
+
     traffic = profiled_traffic() if (args.taxa, args.sites) == (100, 50000) else None
     copy_gbps = G.probe_copy(1 << 30, 5)
     log(f"measured device copy bandwidth: {copy_gbps:.0f} GB/s")
@@ -173,7 +185,7 @@ def main():
             "frac": f32["achieved"] / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": os.path.relpath(PROFILE, ROOT) if traffic else None,
             "traffic_over_algorithmic": traffic / (upd_per_step * f32["bytes_per_update"]) if traffic else None,
-            "kernel": "k_traverse_f32_mfma<20,MODE_LOGL>", "kernel_avg_ms": f32["kernel_avg_ms"],
+            "kernel": f32_kernel_name(f32["tile_waves_per_category"]), "kernel_avg_ms": f32["kernel_avg_ms"],
             "alg_bytes_per_update": f32["bytes_per_update"],
             "measured_copy_GBps": copy_gbps, "frac_of_measured_copy": f32["achieved"] / copy_gbps,
             "tile_waves_per_category": f32["tile_waves_per_category"],

@@ -64,6 +64,7 @@ _SIGS = {
     "plk_last_error": (ctypes.c_char_p, []),
     "plk_version": (_int, []),
     "plk_device_count": (_int, []),
+    "plk_device_units": (_int, []),
     "plk_create": (_int, [ctypes.POINTER(_P), _int, _int, _int, _int, _int, _int, _int]),
     "plk_destroy": (_int, [_P]),
     "plk_memory_required": (ctypes.c_int64, [_int] * 7),
@@ -196,6 +197,10 @@ def _d(a):
 
 def device_count():
     return lib().plk_device_count()
+
+
+def device_units():
+    return lib().plk_device_units()
 
 
 def probe_copy(nbytes=1 << 30, reps=5):

@@ -454,6 +454,13 @@ extern "C" int plk_device_count(void) {
     return n;
 }
 
+extern "C" int plk_device_units(void) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return n;
+}
+
 extern "C" int64_t plk_memory_required(int nstates, int ncat, int ntaxa, int nptn, int nslots, int precision,
                                        int scaling) {
     int64_t pad = round_up(nptn, 64), block = (int64_t)nstates * ncat, rs = precision == PLK_FP64 ? 8 : 4;

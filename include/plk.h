@@ -72,6 +72,10 @@ const char *plk_last_error(void);
 int plk_version(void);
 /* Number of visible GPUs (0 when none); never fails. */
 int plk_device_count(void);
+/* Compute units of the current GPU (0 when none).  The 20-state FP32 traversal runs the
+ * last tiles of a launch whose tile count is just above a multiple of this number as
+ * single 16-pattern-block units; no counterpart in the reference (a device query). */
+int plk_device_units(void);
 
 /* PhyloTree::initializeAllPartialLh (tree/phylotree.cpp:905-960).  nslots directional
  * partial buffers of nptn x ncat x nstates reals each. */
