@@ -24,16 +24,18 @@ namespace dev {
 // or plainly [c][x][y] for the vector-ALU DNA path; leaf tables leaf[s][c][r] =
 // (P_c tip_s)[f32_orow(r / 16, r % 16)], rows f32_trow(N) long (zero padded for the
 // matrix-core forms).
+// Categories c0 .. c1-1 of the job (all by default).
 template <int N>
-__device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j) {
+__device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j, int c0 = 0, int c1 = -1) {
     constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
     const int ncat = P.ncat, su = P.su;
+    if (c1 < 0) c1 = ncat;
     float *M = (float *)P.mats + (int64_t)j.op * P.mat_stride;
     float *F = M + (int64_t)j.side * ncat * FR;
     float *T = M + (int64_t)2 * ncat * FR + (int64_t)j.side * (su + 1) * ncat * TROW;
     __shared__ double ex[N];
     __shared__ double Pm[N * N];
-    for (int c = 0; c < ncat; c++) {
+    for (int c = c0; c < c1; c++) {
         for (int i = threadIdx.x; i < N; i += blockDim.x) ex[i] = exp(P.eval[i] * (P.rates[c] * j.len));
         __syncthreads();
         for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {

@@ -5,9 +5,11 @@ namespace plk {
 namespace {
 using namespace dev;
 
+// one workgroup per (job, category)
 template <int N>
 __global__ void __launch_bounds__(256) k_pmatrix_f32(KParams P, const PJob *__restrict__ jobs) {
-    pmatrix_f32_body<N>(P, jobs[blockIdx.x]);
+    const int c = blockIdx.x % P.ncat;
+    pmatrix_f32_body<N>(P, jobs[blockIdx.x / P.ncat], c, c + 1);
 }
 
 // WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1; WIDE = 1
@@ -126,9 +128,9 @@ hipError_t traverse_f32_n(const KParams &p, int mode, hipStream_t s) {
 
 hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s) {
     switch (p.n) {
-        case 4: hipLaunchKernelGGL(k_pmatrix_f32<4>, dim3(njobs), dim3(256), 0, s, p, jobs); break;
-        case 20: hipLaunchKernelGGL(k_pmatrix_f32<20>, dim3(njobs), dim3(256), 0, s, p, jobs); break;
-        case 61: hipLaunchKernelGGL(k_pmatrix_f32<61>, dim3(njobs), dim3(256), 0, s, p, jobs); break;
+        case 4: hipLaunchKernelGGL(k_pmatrix_f32<4>, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs); break;
+        case 20: hipLaunchKernelGGL(k_pmatrix_f32<20>, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs); break;
+        case 61: hipLaunchKernelGGL(k_pmatrix_f32<61>, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
