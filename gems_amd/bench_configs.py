@@ -162,6 +162,54 @@ def config1(quick):
             "fp32_over_cpu": r32["updates_per_s"] / cpu}
 
 
+def rell_and_newton(p, o, thr, quick):
+    """UFBoot RELL (1000 replicates) and branch-length Newton on the config-2 problem: the
+    device against the oracle's restatements on `thr` host cores (RELL: one core)."""
+    import support as S
+    rng = np.random.default_rng(7)
+    ns = 1000
+    w = p.freq / p.freq.sum()
+    counts = rng.multinomial(int(p.freq.sum()), w, size=ns).astype(np.float32)
+    uni = rng.random(ns)
+    a, b = 0, int(p.nei[0, 0])
+    g = G.Tree(p, precision=G.FP64)
+    lnl = g.loglik(a, b)
+    r = G.Rell(counts)
+    r.save_tree(g.context(), lnl, uni)
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r.save_tree(g.context(), lnl, uni)
+    rell_gpu = (time.perf_counter() - t0) / reps
+    r.close()
+    lo, plo = o.loglik(a, b, with_patterns=True)
+    orl = S.OracleRell(counts)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        orl.save_tree(plo, lo, uniform=uni)
+    rell_cpu = (time.perf_counter() - t0) / 3
+    res = {"rell": {"replicates": ns, "patterns": p.nptn, "gpu_ms_per_tree": rell_gpu * 1e3,
+                    "cpu_oracle_ms_per_tree": rell_cpu * 1e3, "cpu_threads": 1,
+                    "gpu_bytes_per_s_counts": ns * p.nptn * 4 / rell_gpu}}
+    # Newton: all branches once (optimizeAllBranches with one iteration), device loop
+    iters = 1
+    g.full_traversal()
+    t0 = time.perf_counter()
+    lg = g.optimize_all(iterations=iters, tol=0.01)
+    newton_gpu = time.perf_counter() - t0
+    c = g.counters()
+    g.close()
+    res["optimize_all_branches_1_round"] = {"branches": 2 * p.ntaxa - 3, "gpu_ms": newton_gpu * 1e3,
+                                            "derivative_evals": int(c[1]), "logl_gpu": lg}
+    if not quick:
+        t0 = time.perf_counter()
+        lc = o.optimize_all(iterations=iters, tol=0.01)
+        res["optimize_all_branches_1_round"].update({"cpu_oracle_ms": (time.perf_counter() - t0) * 1e3,
+                                                     "cpu_threads": thr, "logl_cpu": lc,
+                                                     "logl_rel_err": abs(lg - lc) / abs(lc)})
+    return res
+
+
 def config2(quick):
     p = G.Problem(G.SEQ_PROTEIN, 100, 50000, mean_len=0.2, alpha=0.8, seed=1)
     log(f"config2: {p.nptn} patterns")
@@ -182,10 +230,12 @@ def config2(quick):
             e = max(e, max(abs(dg[i] - dd[i]) / max(1.0, abs(dd[i])) for i in range(2)))
         errs[name] = e
         g.close()
+    extras = rell_and_newton(p, o, thr, quick)
+    o.close()
     return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
             "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
             "deriv_max_rel_err": errs, "fp32_over_fp64": r32["updates_per_s"] / r64["updates_per_s"],
-            "fp32_over_cpu": r32["updates_per_s"] / cpu}
+            "fp32_over_cpu": r32["updates_per_s"] / cpu, **extras}
 
 
 def config3(quick, nparts=300):
