@@ -140,6 +140,9 @@ _SIGS = {
     "plk_tree_set_gamma_shape": (_int, [_P, _dbl]),
     "plk_tree_get_gamma_shape": (_int, [_P, _dp]),
     "plk_tree_set_gamma_median": (_int, [_P, _int]),
+    "plk_tree_set_gtr": (_int, [_P, _int, _dp, _dp]),
+    "plk_tree_get_gtr": (_int, [_P, _dp]),
+    "plk_tree_optimize_model": (_int, [_P, _dbl, _dp]),
     "plk_tree_optimize_gamma": (_int, [_P, _dbl, _dbl, _dbl, _dp, _dp]),
     "plk_tree_optimize_parameters": (_int, [_P, _int, _dbl, _dbl, _dbl, _dbl, _dbl, _dbl, _int, _dp, _dp]),
     "plk_sim_create_on_tree": (_int, [ctypes.POINTER(_P), _int, _int, _ip, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
@@ -364,6 +367,24 @@ class Tree:
 
     def set_gamma_shape(self, alpha):
         _check(lib().plk_tree_set_gamma_shape(self.h, alpha))
+
+    def set_gtr(self, exch, pi, seq_type=None):
+        exch = np.ascontiguousarray(exch, dtype=np.float64)
+        pi = np.ascontiguousarray(pi, dtype=np.float64)
+        st = self.p.seq_type if seq_type is None else seq_type
+        _check(lib().plk_tree_set_gtr(self.h, st, _d(exch), _d(pi)))
+
+    def gtr(self):
+        n = self.p.n
+        out = np.zeros(n * (n - 1) // 2)
+        _check(lib().plk_tree_get_gtr(self.h, _d(out)))
+        return out
+
+    def optimize_model(self, gradient_epsilon=1e-4):
+        """ModelMarkov::optimizeParameters; returns (exchangeabilities, logL)."""
+        v = ctypes.c_double()
+        _check(lib().plk_tree_optimize_model(self.h, gradient_epsilon, ctypes.byref(v)))
+        return self.gtr(), v.value
 
     def set_gamma_median(self, median):
         _check(lib().plk_tree_set_gamma_median(self.h, int(median)))

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <vector>
 
+#include "bfgs.h"
 #include "brent.h"
 #include "plk.h"
 #include "plk_internal.h"
@@ -39,6 +40,11 @@ class PhyloTreeGPU {
     bool model_stored = false;
     double gamma_shape = 0.0;
     bool gamma_median = false;  // category rates at the medians (RateGamma's cut_median)
+    // Reversible substitution model given by exchangeabilities (upper triangle, row-major)
+    // and fixed frequencies: the eigen system and tip vectors are derived from them.
+    bool gtr_set = false;
+    int seq_type = 0;
+    std::vector<double> m_exch, m_pi;
 
     bool is_leaf(int v) const { return v < ntaxa; }
 
@@ -573,6 +579,59 @@ class PhyloTreeGPU {
         return rc;
     }
 
+    // ModelMarkov::decomposeRateMatrix (model/modelmarkov.cpp:1389-1635) for the stored
+    // exchangeabilities, then the tip vectors; the model is re-uploaded (partials dirty).
+    int decompose_gtr() {
+        const int n = nstates;
+        std::vector<double> Q((size_t)n * n), ev(n), U((size_t)n * n), Ui((size_t)n * n);
+        int rc = plk_rate_matrix(n, m_exch.data(), m_pi.data(), Q.data());
+        if (!rc) rc = plk_eigen(n, Q.data(), m_pi.data(), ev.data(), U.data(), Ui.data());
+        if (rc) return rc;
+        std::vector<double> tip((size_t)(su + 1) * n);
+        if ((rc = plk_tip_lh(n, seq_type, Ui.data(), tip.data()))) return rc;
+        m_eval = ev;
+        m_evec = U;
+        m_inv = Ui;
+        m_tip = tip;
+        return upload_model();
+    }
+
+    // ModelMarkov::targetFunk (model/modelmarkov.cpp:1091-1113): the free exchangeabilities
+    // (all but the last, which stays 1) from x; a change re-decomposes Q; -logL.
+    int gtr_function(const std::vector<double> &x, double &f) {
+        bool changed = false;
+        for (size_t i = 0; i < x.size(); i++) changed |= m_exch[i] != x[i];
+        int rc;
+        if (changed) {
+            std::copy(x.begin(), x.end(), m_exch.begin());
+            if ((rc = decompose_gtr())) return rc;
+        }
+        double lnl = 0.0;
+        rc = loglik(0, nei[0], &lnl, nullptr);
+        f = -lnl;
+        return rc;
+    }
+
+    // ModelMarkov::optimizeParameters (model/modelmarkov.cpp:1166-1243) with fixed
+    // frequencies: BFGS (Optimization::minimizeMultiDimen, utils/optimization.cpp:750-906)
+    // over the free exchangeabilities in [MIN_RATE, MAX_RATE] = [1e-4, 100] with gradient
+    // tolerance max(gradient_epsilon, TOL_RATE = 1e-4) (model/modelmarkov.h:30-32); no
+    // bound is checked, so the search is not restarted.  The model is left at the result.
+    int optimize_gtr(double gradient_epsilon, double &logl) {
+        if (!gtr_set) return fail(PLK_ERR_STATE, "plk_tree_optimize_model: no exchangeabilities set");
+        const int ndim = (int)m_exch.size() - 1;
+        if (ndim < 1) return fail(PLK_ERR_STATE, "plk_tree_optimize_model: nothing to optimise");
+        std::vector<double> x(m_exch.begin(), m_exch.begin() + ndim), lo(ndim, 1e-4), hi(ndim, 100.0);
+        auto f = [this](const std::vector<double> &v, double &fv) { return gtr_function(v, fv); };
+        double fret = 0.0;
+        int rc = bfgs_minimize(f, x, lo, hi, std::max(gradient_epsilon, 1e-4), fret);
+        if (rc) return rc;
+        double fx = 0.0;
+        rc = gtr_function(x, fx);  // getVariables: the model at the returned point
+        logl = -fx;
+        return rc;
+    }
+
     // RateGamma::optimizeParameters(gradient_epsilon, min_gamma, max_gamma)
     // (model/rategamma.cpp:215-232): minimizeOneDimen over the shape with tolerance
     // max(gradient_epsilon, TOL_GAMMA_SHAPE = 0.001) (utils/tools.h:607); the model is left
@@ -591,22 +650,27 @@ class PhyloTreeGPU {
     }
 
     // ModelFactory::optimizeParameters (model/modelfactory.cpp:1558-1720) with branch
-    // lengths optimised (BRLEN_OPTIMIZE), a substitution model with nothing free
-    // (ModelSubst::optimizeParameters adds nothing: optimizeParametersOnly returns the rate
-    // model's logL, 1263-1315) and a Gamma rate model: alternate all-branch Newton passes
-    // (min(i, 3) rounds) with the shape until an iteration gains no more than
-    // logl_epsilon, then a final optimizeAllBranches(100).  RateGamma::rescaleRates is the
-    // base class's 1.0 (model/rateheterogeneity.h:288), so no length rescaling follows.
+    // lengths optimised (BRLEN_OPTIMIZE), the exchangeabilities (when set with
+    // plk_tree_set_gtr) and a Gamma rate model (when a shape is set): alternate
+    // all-branch Newton passes (min(i, 3) rounds) with the model parameters until an
+    // iteration gains no more than logl_epsilon, then a final optimizeAllBranches(100).
+    // RateGamma::rescaleRates is the base class's 1.0 (model/rateheterogeneity.h:288), so
+    // no length rescaling follows.
     int optimize_parameters(int num_iterations, double logl_epsilon, double gradient_epsilon, double min_gamma,
                             double max_gamma, double min_bl, double max_bl, int max_nr, double &out) {
-        if (!(gamma_shape > 0.0)) return fail(PLK_ERR_STATE, "plk_tree_optimize_parameters: no Gamma shape set");
+        const bool free_model = gtr_set && m_exch.size() > 1, free_rates = gamma_shape > 0.0;
+        if (!free_model && !free_rates)
+            return fail(PLK_ERR_STATE, "plk_tree_optimize_parameters: neither exchangeabilities nor a Gamma shape set");
         double cur = 0.0;
         int rc = loglik(0, nei[0], &cur, nullptr);
         if (rc) return rc;
         for (int i = 2; i < num_iterations; i++) {
             double new_lh = 0.0;
             if ((rc = optimize_all(std::min(i, 3), logl_epsilon, min_bl, max_bl, max_nr, new_lh))) return rc;
-            if ((rc = optimize_gamma(gradient_epsilon, min_gamma, max_gamma, new_lh))) return rc;
+            // optimizeParametersOnly (1263-1315): the substitution model, then the rates;
+            // the logL is the rate model's when it has one
+            if (free_model && (rc = optimize_gtr(gradient_epsilon, new_lh))) return rc;
+            if (free_rates && (rc = optimize_gamma(gradient_epsilon, min_gamma, max_gamma, new_lh))) return rc;
             if (new_lh > cur + logl_epsilon) {
                 cur = new_lh;
             } else {
@@ -682,12 +746,39 @@ extern "C" int plk_tree_set_model(plk_tree *t, const double *eval, const double 
     t->m_tip.assign(tip_lh, tip_lh + (size_t)(su + 1) * n);
     t->model_stored = true;
     t->gamma_shape = 0.0;
+    t->gtr_set = false;
     return PLK_OK;
 }
 
 extern "C" int plk_tree_set_gamma_shape(plk_tree *t, double alpha) {
     if (!t || !(alpha > 0.0) || !std::isfinite(alpha)) return fail(PLK_ERR_ARG, "plk_tree_set_gamma_shape: bad arguments");
     return t->set_gamma(alpha);
+}
+
+extern "C" int plk_tree_set_gtr(plk_tree *t, int seq_type, const double *exch, const double *pi) {
+    if (!t || !exch || !pi) return fail(PLK_ERR_ARG, "plk_tree_set_gtr: bad arguments");
+    if (!t->model_stored) return fail(PLK_ERR_STATE, "plk_tree_set_gtr: set a model (rates, proportions) first");
+    const int n = t->nstates, ne = n * (n - 1) / 2;
+    for (int i = 0; i < ne; i++)
+        if (!(exch[i] > 0.0) || !std::isfinite(exch[i])) return fail(PLK_ERR_ARG, "plk_tree_set_gtr: bad exchangeability");
+    for (int i = 0; i < n; i++)
+        if (!(pi[i] > 0.0)) return fail(PLK_ERR_ARG, "plk_tree_set_gtr: frequencies must be positive");
+    t->seq_type = seq_type;
+    t->m_exch.assign(exch, exch + ne);
+    t->m_pi.assign(pi, pi + n);
+    t->gtr_set = true;
+    return t->decompose_gtr();
+}
+
+extern "C" int plk_tree_get_gtr(plk_tree *t, double *exch) {
+    if (!t || !exch || !t->gtr_set) return fail(PLK_ERR_ARG, "plk_tree_get_gtr: bad arguments");
+    std::copy(t->m_exch.begin(), t->m_exch.end(), exch);
+    return PLK_OK;
+}
+
+extern "C" int plk_tree_optimize_model(plk_tree *t, double gradient_epsilon, double *logl) {
+    if (!t || !logl || !(gradient_epsilon >= 0.0)) return fail(PLK_ERR_ARG, "plk_tree_optimize_model: bad arguments");
+    return t->optimize_gtr(gradient_epsilon, *logl);
 }
 
 extern "C" int plk_tree_set_gamma_median(plk_tree *t, int median) {

@@ -102,6 +102,21 @@ int plk_tree_get_gamma_shape(plk_tree *t, double *alpha);
  * the category medians (plk_gamma_rates_median; RateGamma's cut_median,
  * model/rategamma.cpp:111-126) instead of the means */
 int plk_tree_set_gamma_median(plk_tree *t, int median);
+/* Reversible substitution model from exchangeabilities (n(n-1)/2, upper triangle row by
+ * row, e.g. AC AG AT CG CT GT for DNA) and fixed frequencies pi: Q (plk_rate_matrix), its
+ * eigen system (plk_eigen) and the tip vectors of seq_type (plk_tip_lh) replace the
+ * model's; rates and proportions stay.  Needs plk_tree_set_model first (which clears
+ * it).  The last exchangeability is the reference one (held at its value by the search). */
+int plk_tree_set_gtr(plk_tree *t, int seq_type, const double *exch, const double *pi);
+int plk_tree_get_gtr(plk_tree *t, double *exch);
+/* ModelMarkov::optimizeParameters (model/modelmarkov.cpp:1166-1243) with fixed
+ * frequencies: BFGS with forward-difference gradients (Optimization::minimizeMultiDimen /
+ * dfpmin / lnsrch / derivativeFunk, utils/optimization.cpp:639-939) over the first
+ * n(n-1)/2 - 1 exchangeabilities in [MIN_RATE, MAX_RATE] = [1e-4, 100], gradient tolerance
+ * max(gradient_epsilon, TOL_RATE = 1e-4) (model/modelmarkov.h:30-32); every evaluation
+ * re-decomposes Q on the host and runs a full traversal on the device.  Leaves the model
+ * at the result; logl receives its logL. */
+int plk_tree_optimize_model(plk_tree *t, double gradient_epsilon, double *logl);
 /* RateGamma::optimizeParameters(gradient_epsilon, min_gamma, max_gamma)
  * (model/rategamma.cpp:215-232): Brent's method behind Optimization::minimizeOneDimen
  * (utils/optimization.cpp:174-372) over the shape, starting at the current one, with
@@ -112,9 +127,11 @@ int plk_tree_set_gamma_median(plk_tree *t, int median);
 int plk_tree_optimize_gamma(plk_tree *t, double gradient_epsilon, double min_gamma, double max_gamma, double *alpha,
                             double *logl);
 /* ModelFactory::optimizeParameters(BRLEN_OPTIMIZE, ..., logl_epsilon, gradient_epsilon)
- * (model/modelfactory.cpp:1558-1720) for a substitution model with no free parameters
- * and a Gamma rate model: for i = 2 .. num_param_iterations-1, optimizeAllBranches(min(i,
- * 3), logl_epsilon) then the shape (plk_tree_optimize_gamma); stop when an iteration's
+ * (model/modelfactory.cpp:1558-1720) for the exchangeabilities (when set with
+ * plk_tree_set_gtr) and a Gamma rate model (when a shape is set): for i = 2 ..
+ * num_param_iterations-1, optimizeAllBranches(min(i, 3), logl_epsilon), then the
+ * exchangeabilities (plk_tree_optimize_model), then the shape (plk_tree_optimize_gamma);
+ * stop when an iteration's
  * logL is not above the previous one + logl_epsilon, then optimizeAllBranches(100,
  * logl_epsilon).  The reference's defaults: num_param_iterations 100, max_nr_step 100. */
 int plk_tree_optimize_parameters(plk_tree *t, int num_param_iterations, double logl_epsilon, double gradient_epsilon,

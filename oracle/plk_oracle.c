@@ -151,6 +151,16 @@ void orc_set_model(orc_tree *t, const double *eval, const double *evec, const do
     orc_clear_all(t);
 }
 
+/* a new eigen system (decomposeRateMatrix of a new Q); rates, proportions and the Gamma
+ * shape stay */
+void orc_set_eigen(orc_tree *t, const double *eval, const double *evec, const double *inv_evec) {
+    int n = t->n;
+    memcpy(t->eval, eval, sizeof(double) * n);
+    memcpy(t->evec, evec, sizeof(double) * n * n);
+    memcpy(t->inv_evec, inv_evec, sizeof(double) * n * n);
+    orc_clear_all(t);
+}
+
 void orc_set_tip_lh(orc_tree *t, int state_unknown, const double *tip_lh) {
     free(t->tip_lh);
     t->su = state_unknown;

@@ -29,6 +29,8 @@
  *   Gamma shape optimisation      RateGamma::optimizeParameters model/rategamma.cpp:178-232
  *   model-parameter loop          ModelFactory::optimizeParameters model/modelfactory.cpp:1263-1315,
  *                                 1558-1720
+ *   (BFGS for the exchangeabilities, ModelMarkov::optimizeParameters, is restated in
+ *   oracle/optim_oracle.py and driven from tests/support.py)
  *
  * Memory layout follows the reference's pattern-interleaved SIMD layout with
  * ORC_V lanes: element (ptn, c, x) of a partial sits at double offset
@@ -77,6 +79,8 @@ void orc_destroy(orc_tree *t);
 
 void orc_set_model(orc_tree *t, const double *eval, const double *evec,
                    const double *inv_evec, const double *rates, const double *props);
+/* replace the eigen system only (rates, proportions and the Gamma shape stay) */
+void orc_set_eigen(orc_tree *t, const double *eval, const double *evec, const double *inv_evec);
 /* tip_lh: (state_unknown+1) x nstates, already multiplied by the inverse eigenvectors */
 void orc_set_tip_lh(orc_tree *t, int state_unknown, const double *tip_lh);
 /* states: ntaxa x nptn (row per taxon); freq, invar: nptn */

@@ -81,6 +81,7 @@ def oracle_lib(lanes=None):
         "orc_minimize_one_dimen": (ctypes.c_double, [ORC_FN1, P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                      ctypes.c_double, dp, dp]),
         "orc_set_gamma_shape": (None, [P, ctypes.c_double]),
+        "orc_set_eigen": (None, [P, dp, dp, dp]),
         "orc_get_gamma_shape": (ctypes.c_double, [P]),
         "orc_optimize_gamma": (ctypes.c_double, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_optimize_parameters": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -181,6 +182,57 @@ class Oracle:
     def set_gamma_median(self, median):
         self.lib.orc_set_gamma_median(self.h, int(median))
 
+    # ---- reversible model from exchangeabilities (ModelMarkov), for the model search
+    def set_gtr(self, exch, pi):
+        self.gtr_exch = np.array(exch, dtype=np.float64)
+        self.gtr_pi = np.array(pi, dtype=np.float64)
+        self._apply_gtr()
+
+    def _apply_gtr(self):
+        n = self.p.n
+        Q = rate_matrix(self.gtr_exch, self.gtr_pi)
+        w, U, Ui = eigen_system(Q, self.gtr_pi)
+        w, U, Ui = (np.ascontiguousarray(a, dtype=np.float64) for a in (w, U, Ui))
+        self.lib.orc_set_eigen(self.h, _dp(w), _dp(U), _dp(Ui))
+        su = self.lib.orc_state_unknown(n, self.p.seq_type)
+        tip = np.zeros((su + 1) * n)
+        self.lib.orc_tip_lh(n, self.p.seq_type, _dp(Ui), _dp(tip))
+        self.lib.orc_set_tip_lh(self.h, su, _dp(tip))
+
+    def gtr_objective(self, x):
+        """ModelMarkov::targetFunk: the free exchangeabilities from x, -logL."""
+        nd = len(x)
+        if any(self.gtr_exch[i] != x[i] for i in range(nd)):
+            self.gtr_exch[:nd] = x
+            self._apply_gtr()
+        return -self.compute_likelihood()
+
+    def optimize_model(self, gradient_epsilon=1e-4):
+        """ModelMarkov::optimizeParameters with fixed frequencies (BFGS of optim_oracle)."""
+        nd = len(self.gtr_exch) - 1
+        x = [float(v) for v in self.gtr_exch[:nd]]
+        optim_oracle().dfpmin(self.gtr_objective, x, [1e-4] * nd, [100.0] * nd, max(gradient_epsilon, 1e-4))
+        f = self.gtr_objective(x)
+        return self.gtr_exch.copy(), -f
+
+    def optimize_parameters_model(self, num_iterations=100, logl_epsilon=0.01, gradient_epsilon=1e-4,
+                                  min_bl=1e-6, max_bl=10.0, max_nr=100, gamma=True):
+        """ModelFactory::optimizeParameters with the exchangeabilities free (and the Gamma
+        shape when gamma), in Python over the oracle's pieces."""
+        cur = self.compute_likelihood()
+        for i in range(2, num_iterations):
+            self.optimize_all(iterations=min(i, 3), tol=logl_epsilon, min_bl=min_bl, max_bl=max_bl, max_nr=max_nr)
+            _e, new = self.optimize_model(gradient_epsilon)
+            if gamma:
+                _a, new = self.optimize_gamma(gradient_epsilon)
+            if new > cur + logl_epsilon:
+                cur = new
+            else:
+                cur = self.optimize_all(iterations=100, tol=logl_epsilon, min_bl=min_bl, max_bl=max_bl,
+                                        max_nr=max_nr)
+                break
+        return self.gtr_exch.copy(), cur
+
     def compute_likelihood(self):
         return self.loglik(0, int(self.p.nei[0][0]))
 
@@ -193,6 +245,20 @@ class Oracle:
         v = self.lib.orc_optimize_parameters(self.h, num_iterations, logl_epsilon, gradient_epsilon, min_gamma,
                                              max_gamma, min_bl, max_bl, max_nr)
         return self.gamma_shape(), v
+
+
+_optim = None
+
+
+def optim_oracle():
+    """oracle/optim_oracle.py (the BFGS restatement)."""
+    global _optim
+    if _optim is None:
+        import importlib.util
+        spec = importlib.util.spec_from_file_location("optim_oracle", ORACLE_DIR / "optim_oracle.py")
+        _optim = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(_optim)
+    return _optim
 
 
 def minimize_one_dimen(fn, xmin, xguess, xmax, tol):

@@ -2,6 +2,7 @@
 (plk_tree_optimize_gamma, RateGamma::optimizeParameters) and the branch/shape
 alternation (plk_tree_optimize_parameters, ModelFactory::optimizeParameters), every
 evaluation a device traversal, against the oracle's restatement on the same inputs."""
+import numpy as np
 import pytest
 
 import gems_amd as G
@@ -163,3 +164,62 @@ def test_gamma_needs_a_shape_and_model():
     g.set_model(p)  # a new model clears the shape
     assert g.gamma_shape() == 0.0
     g.close()
+
+
+@pytest.mark.parametrize("prec", [G.FP64, G.FP32], ids=["f64", "f32"])
+def test_optimize_model_matches_oracle(prec):
+    """Exchangeabilities by BFGS (plk_tree_optimize_model): the device search step for
+    step against the oracle's dfpmin driven by the device's own logL, logL(exch) against the
+    oracle, and the end points against the oracle's own search."""
+    p = G.Problem(G.SEQ_DNA, 14, 5000, mean_len=0.2, alpha=0.8, seed=43)
+    say("optimize_model", prec, "nptn", p.nptn)
+    g = G.Tree(p, precision=prec)
+    o = S.Oracle(p, nthreads=4)
+    for t in (g, o):
+        t.set_gamma_shape(0.8)
+        t.set_gtr(np.ones(6), p.pi)
+    assert rel(g.compute_likelihood(), o.compute_likelihood()) <= logl_tol(prec)
+    eg, lg = g.optimize_model(gradient_epsilon=1e-4)
+    eo, lo = o.optimize_model(gradient_epsilon=1e-4)
+    say(f"  exch gpu {np.round(eg, 5)} oracle {np.round(eo, 5)}  logL gpu {lg:.8f} oracle {lo:.8f}")
+
+    def f_dev(x):
+        g.set_gtr(np.r_[x, 1.0], p.pi)
+        return -g.compute_likelihood()
+
+    x = [1.0] * 5
+    g.set_gtr(np.ones(6), p.pi)
+    S.optim_oracle().dfpmin(f_dev, x, [1e-4] * 5, [100.0] * 5, 1e-4)
+    np.testing.assert_allclose(x, eg[:5], rtol=1e-9)
+    o.set_gtr(eg, p.pi)
+    assert rel(f_dev(list(eg[:5])), -o.compute_likelihood()) <= logl_tol(prec)
+    if prec == G.FP64:
+        np.testing.assert_allclose(eg, eo, rtol=1e-4)
+        assert abs(lg - lo) <= 1e-4
+    else:
+        # FP32 logL carries ~1e-8 relative rounding, which forward differences with steps of
+        # 1e-4 relative turn into gradient noise: the search stops a little short
+        np.testing.assert_allclose(eg, eo, rtol=5e-2)
+        assert lo - 0.5 <= lg <= lo + 1e-3
+    g.close()
+    o.close()
+
+
+def test_optimize_parameters_with_model_matches_oracle():
+    p = G.Problem(G.SEQ_DNA, 12, 3000, mean_len=0.2, alpha=0.8, seed=47)
+    say("optimize_parameters with exchangeabilities", "nptn", p.nptn)
+    g = G.Tree(p)
+    o = S.Oracle(p, nthreads=4)
+    for t in (g, o):
+        t.set_gamma_shape(1.5)
+        t.set_gtr(np.ones(6), p.pi)
+    start = g.compute_likelihood()
+    ag, lg = g.optimize_parameters(logl_epsilon=0.01, gradient_epsilon=1e-4)
+    eo, lo = o.optimize_parameters_model(logl_epsilon=0.01, gradient_epsilon=1e-4)
+    eg = g.gtr()
+    say(f"  alpha gpu {ag:.6f} oracle {o.gamma_shape():.6f}  logL gpu {lg:.6f} oracle {lo:.6f}")
+    assert lg > start
+    assert abs(lg - lo) <= 0.01 and rel(ag, o.gamma_shape()) <= 2 * ALPHA_END
+    np.testing.assert_allclose(eg, eo, rtol=3e-2)
+    g.close()
+    o.close()

@@ -152,3 +152,74 @@ def test_parameter_loop_ends_stationary(dna_problem):
             assert df * df / (2 * abs(ddf)) < 0.01 or (df < 0 and t < 1e-4)
     only_branches.close()
     o.close()
+
+
+# ---- BFGS (ModelMarkov::optimizeParameters) -----------------------------------------
+
+def test_bfgs_quadratic_bowl_and_scipy():
+    a = np.array([1.0, 4.0, 0.5, 2.0, 8.0])
+    c = np.array([0.7, 2.0, 5.0, 1.3, 0.2])
+    fn = lambda x: float(np.sum(a * (np.asarray(x) - c) ** 2) + 3.0)  # noqa: E731
+    x = [1.0] * 5
+    fret = S.optim_oracle().dfpmin(fn, x, [1e-4] * 5, [100.0] * 5, 1e-6)
+    # forward differences with h = 1e-4 |x| shift the stationary point by ~h / 2
+    np.testing.assert_allclose(x, c, rtol=2e-3, atol=1e-4)
+    ref = scipy.optimize.minimize(fn, np.ones(5), method="BFGS")
+    assert fret == pytest.approx(fn(x)) and fret <= ref.fun + 1e-6
+
+
+def test_bfgs_rosenbrock_valley():
+    fn = lambda x: 100.0 * (x[1] - x[0] ** 2) ** 2 + (1.0 - x[0]) ** 2  # noqa: E731
+    x = [0.5, 1.5]
+    fret = S.optim_oracle().dfpmin(fn, x, [-10.0, -10.0], [10.0, 10.0], 1e-8)
+    # forward-difference gradients stop the search along the curved valley floor a few
+    # percent short of (1, 1)
+    np.testing.assert_allclose(x, [1.0, 1.0], atol=0.06)
+    # (fret can be the value of the last rejected trial point, as in the reference's lnsrch
+    # when the step underflows)
+    assert fn(x) < 1e-3 and fret < 1e-2
+
+
+def test_bfgs_minimum_on_a_bound():
+    fn = lambda x: float(sum((v + 1.0) ** 2 for v in x))  # noqa: E731
+    x = [2.0, 3.0, 0.5]
+    S.optim_oracle().dfpmin(fn, x, [0.0] * 3, [10.0] * 3, 1e-8)
+    np.testing.assert_allclose(x, [0.0, 0.0, 0.0], atol=1e-6)
+
+
+def test_bfgs_first_evaluations_are_forward_differences():
+    calls = []
+
+    def fn(x):
+        calls.append(list(x))
+        return float(sum(v * v for v in x))
+
+    x = [2.0, 0.0, 5.0]
+    S.optim_oracle().dfpmin(fn, x, [-10.0] * 3, [10.0] * 3, 1e-8)
+    assert calls[0] == [2.0, 0.0, 5.0]
+    assert calls[1] == [2.0 + 2e-4, 0.0, 5.0]
+    assert calls[2] == [2.0, 1e-4, 5.0]      # h = ERROR_X when x_d == 0
+    assert calls[3] == [2.0, 0.0, 5.0 + 5e-4]
+
+
+def true_exchangeabilities(p):
+    """Relative exchangeabilities of the simulating Q (Q_ij = r_ij pi_j s), GT = 1."""
+    Q, pi = p.Q, p.pi
+    e = np.array([Q[i, j] / pi[j] for i in range(4) for j in range(i + 1, 4)])
+    return e / e[-1]
+
+
+def test_gtr_search_recovers_the_simulating_rates():
+    p = G.Problem(G.SEQ_DNA, 12, 6000, mean_len=0.2, alpha=0.8, ncat=4, seed=17)
+    o = S.Oracle(p, nthreads=2)
+    o.set_gamma_shape(0.8)
+    o.set_gtr(np.ones(6), p.pi)
+    start = o.compute_likelihood()
+    ex, lnl = o.optimize_model(gradient_epsilon=1e-4)
+    assert lnl > start
+    np.testing.assert_allclose(ex / ex[-1], true_exchangeabilities(p), rtol=0.25)
+    # a stationary point: scipy's bounded quasi-Newton from there gains next to nothing
+    f = lambda x: o.gtr_objective(list(x))  # noqa: E731
+    ref = scipy.optimize.minimize(f, ex[:5], method="L-BFGS-B", bounds=[(1e-4, 100.0)] * 5)
+    assert -ref.fun <= lnl + 0.05
+    o.close()
