@@ -101,12 +101,15 @@ def nni_cpu(p, ib, threads):
     return sc, el
 
 
-def param_loop(p, precision, threads=None):
-    """ModelFactory::optimizeParameters (branch lengths + Gamma shape from 1.0) on the
-    device, the oracle's restatement timed beside it on `threads` host cores."""
+def param_loop(p, precision, threads=None, gtr=False):
+    """ModelFactory::optimizeParameters (branch lengths + Gamma shape from 1.0, and with gtr
+    the exchangeabilities from all ones) on the device, the oracle's restatement timed
+    beside it on `threads` host cores."""
     import support as S
     g = G.Tree(p, precision=precision)
     g.set_gamma_shape(1.0)
+    if gtr:
+        g.set_gtr(np.ones(6), p.pi)
     c0 = g.counters()
     t0 = time.perf_counter()
     ag, lg = g.optimize_parameters(logl_epsilon=0.01, gradient_epsilon=1e-4)
@@ -119,7 +122,12 @@ def param_loop(p, precision, threads=None):
         o = S.Oracle(p, nthreads=threads)
         o.set_gamma_shape(1.0)
         t0 = time.perf_counter()
-        ao, lo = o.optimize_parameters(logl_epsilon=0.01, gradient_epsilon=1e-4)
+        if gtr:
+            o.set_gtr(np.ones(6), p.pi)
+            o.optimize_parameters_model(logl_epsilon=0.01, gradient_epsilon=1e-4)
+            ao, lo = o.gamma_shape(), o.compute_likelihood()
+        else:
+            ao, lo = o.optimize_parameters(logl_epsilon=0.01, gradient_epsilon=1e-4)
         res.update({"cpu_oracle_ms": (time.perf_counter() - t0) * 1e3, "cpu_threads": threads, "cpu_alpha": ao,
                     "cpu_logl": lo, "logl_rel_err": abs(lg - lo) / abs(lo), "alpha_rel_err": abs(ag - ao) / ao})
         o.close()
@@ -148,7 +156,8 @@ def config1(quick):
            "score_max_rel_err_fp64": float(np.max(np.abs(n64 - nco) / np.abs(nco))),
            "score_max_rel_err_fp32": float(np.max(np.abs(n32 - nco) / np.abs(nco))),
            "best_move_agrees_fp64": bool(np.array_equal(np.argmax(n64, axis=1), np.argmax(nco, axis=1)))}
-    params = {"fp64": param_loop(p, G.FP64, thr), "fp32": param_loop(p, G.FP32)}
+    params = {"fp64": param_loop(p, G.FP64, thr), "fp32": param_loop(p, G.FP32),
+              "fp64_gtr": param_loop(p, G.FP64, thr, gtr=True)}
     return {"patterns": p.nptn, "fp64": r64, "fp32": r32, "cpu_updates_per_s": cpu, "cpu_threads": thr,
             "optimize_parameters": params,
             "logl_rel_err_fp64": abs(r64["logl"] - ref) / abs(ref), "logl_rel_err_fp32": abs(r32["logl"] - ref) / abs(ref),
