@@ -69,13 +69,19 @@ class PhyloTreeGPU {
         }
     }
 
+    // subtree sizes by direction (dad's neighbour slot of node), -1 while unknown; the
+    // topology changes only in plk_tree_set_topology, which clears them
+    mutable std::vector<int> ssize;
     int subtree_size(int node, int dad) const {
         if (is_leaf(node)) return 1;
+        const int d = dad >= 0 ? dir(dad, node) : -1;
+        if (d >= 0 && ssize[d] >= 0) return ssize[d];
         int s = 0;
         for (int k = 0; k < 3; k++) {
             int x = nei[node * 3 + k];
             if (x >= 0 && x != dad) s += subtree_size(x, node);
         }
+        if (d >= 0) ssize[d] = s;
         return s;
     }
 
@@ -703,6 +709,7 @@ extern "C" int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int nta
     t->nstates = nstates;
     t->ncat = ncat;
     t->nei.assign((size_t)t->nnodes * 3, -1);
+    t->ssize.assign((size_t)t->nnodes * 3, -1);
     t->len.assign((size_t)t->nnodes * 3, 0.0);
     t->orient.assign(t->nnodes, -1);
     t->computed.assign(t->all_dirs ? (size_t)3 * (ntaxa - 2) : (size_t)t->nnodes, 0);
@@ -845,6 +852,7 @@ extern "C" int plk_tree_set_topology(plk_tree *t, const int32_t *nei, const doub
             return fail(PLK_ERR_ARG, "plk_tree_set_topology: tree must be binary with leaves 0..ntaxa-1");
     }
     t->nei.assign(nei, nei + (size_t)t->nnodes * 3);
+    std::fill(t->ssize.begin(), t->ssize.end(), -1);
     t->len.assign(len, len + (size_t)t->nnodes * 3);
     std::fill(t->orient.begin(), t->orient.end(), -1);
     t->clear_all();
