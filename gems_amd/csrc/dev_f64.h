@@ -20,6 +20,150 @@ namespace dev {
 // op, echild[c][x][i] = U[x][i] exp(lambda_i r_c t) (held as [c][i][x] for n <= 20), and
 // for a leaf child the table
 // leaf[s][c][x] = sum_i echild[c][x][i] tip_lh[s][i] with the unknown-state row = 1.
+// ---------------------------------------------------------------------------------------
+// 61-state (codon) op with its matrices in LDS.  A 61-double column does not fit the
+// scalar registers, and one scalar load per element leaves the single wavefront of a SIMD
+// waiting on nearly every FMA.  Here the workgroup copies the op's matrices into LDS,
+// one phase at a time: the right child's echild of every category as [c][i][x], then the
+// left's, then U^-1 as [j][x], rows padded to 62 doubles so that every row starts 16-byte
+// aligned.  Each wavefront (one category) reads them as broadcast 16-byte LDS loads,
+// with its child's values streamed from memory a block of 8 ahead.  Same sums in the
+// same order as op_partial, so the stored values are identical.  ncat <= 4.
+constexpr int kRow61 = 62;                  // padded row length (doubles)
+constexpr int kLds61 = 4 * 61 * kRow61;     // doubles: four categories' matrices
+
+// echild of one side for every category, row-major [c][x][i] in memory, into E as [c][i][x]
+__device__ __forceinline__ void stage61_echild(const double *__restrict__ src, int ncat, double *E) {
+    __syncthreads();  // every wavefront is done with the previous contents
+    for (int idx = threadIdx.x; idx < ncat * 61 * 61; idx += blockDim.x) {
+        const int c = idx / (61 * 61), rem = idx - c * 61 * 61, x = rem / 61, i = rem - x * 61;
+        E[(c * 61 + i) * kRow61 + x] = src[idx];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void stage61_uinv(const double *__restrict__ uinv, double *E) {
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < 61 * 61; idx += blockDim.x) {
+        const int j = idx / 61, x = idx - j * 61;
+        E[j * kRow61 + x] = uinv[idx];
+    }
+    __syncthreads();
+}
+
+// acc[x] = sum_i Ec[i][X0 + x] v[i] over i = 0..60 in ascending order, v[i] = V[i * 64];
+// X0 is even, so a row segment starts 16-byte aligned; an odd NX reads one padding value.
+template <int X0, int NX>
+__device__ __forceinline__ void cols61(const double *Ec, const double *__restrict__ V, double (&acc)[NX]) {
+#pragma unroll
+    for (int x = 0; x < NX; x++) acc[x] = 0.0;
+    double nxt[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) nxt[u] = V[u * 64];
+    for (int i0 = 0; i0 < 61; i0 += 8) {
+        double cur[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) cur[u] = nxt[u];
+        // the next block (indices clamped into the partial; values past 60 are not used)
+#pragma unroll
+        for (int u = 0; u < 8; u++) nxt[u] = V[min(i0 + 8 + u, 60) * 64];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            if (i0 + u >= 61) break;
+            const double2 *row = (const double2 *)(Ec + (i0 + u) * kRow61 + X0);
+#pragma unroll
+            for (int x = 0; x < NX; x += 2) {
+                const double2 e = row[x / 2];
+                acc[x] = fma(e.x, cur[u], acc[x]);
+                if (x + 1 < NX) acc[x + 1] = fma(e.y, cur[u], acc[x + 1]);
+            }
+        }
+    }
+}
+
+// out[j] = sum_x Ui[j][x] t[x] (ascending x), four rows at a time, each value stored as
+// formed; returns max_j |out[j]|.
+__device__ __forceinline__ double rows61(const double *Ui, const double (&t)[61], double *__restrict__ dst) {
+    double m = 0.0;
+    for (int j0 = 0; j0 < 60; j0 += 4) {
+        double o[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int x = 0; x < 61; x += 2) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double2 e = ((const double2 *)(Ui + (j0 + q) * kRow61))[x / 2];
+                o[q] = fma(e.x, t[x], o[q]);
+                if (x + 1 < 61) o[q] = fma(e.y, t[x + 1], o[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            m = fmax(m, fabs(o[q]));
+            dst[(j0 + q) * 64] = o[q];
+        }
+    }
+    double o = 0.0;
+#pragma unroll
+    for (int x = 0; x < 61; x += 2) {
+        const double2 e = ((const double2 *)(Ui + 60 * kRow61))[x / 2];
+        o = fma(e.x, t[x], o);
+        if (x + 1 < 61) o = fma(e.y, t[x + 1], o);
+    }
+    dst[60 * 64] = o;
+    return fmax(m, fabs(o));
+}
+
+// One op for category c of this wavefront (mine = c < ncat); every wavefront of the
+// workgroup calls it (the staging has barriers).  Returns max_j |out[j]| (0 if not mine).
+__device__ __forceinline__ double op_partial_lds61(const KParams &P, const DevOp &op, const double *__restrict__ M,
+                                                   bool mine, int c, int64_t off, int sl, int sr,
+                                                   double *__restrict__ dst, double *E) {
+    constexpr int N = 61;
+    const int ncat = P.ncat, block = N * ncat;
+    const double *__restrict__ part = (const double *)P.part;
+    const double *__restrict__ tabs = M + (int64_t)2 * ncat * N * N;
+    const double *Ec = E + c * 61 * kRow61;
+    double t[N];
+    if (op.kind == KIND_TT) {
+        if (mine) {
+            const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
+            const double *__restrict__ tr = tabs + (int64_t)(P.su + 1) * block + (int64_t)sr * block + c * N;
+#pragma unroll
+            for (int x = 0; x < N; x++) t[x] = tl[x] * tr[x];
+        }
+    } else {
+        stage61_echild(M + (int64_t)ncat * N * N, ncat, E);  // right child's echild
+        if (mine) {
+            cols61<0, 61>(Ec, part + (int64_t)op.right * P.slot_stride + off, t);
+            if (op.kind == KIND_TI) {
+                const double *__restrict__ tl = tabs + (int64_t)sl * block + c * N;
+#pragma unroll
+                for (int x = 0; x < N; x++) t[x] = tl[x] * t[x];
+            }
+        }
+        if (op.kind == KIND_II) {
+            stage61_echild(M, ncat, E);  // left child's echild
+            if (mine) {
+                const double *__restrict__ L = part + (int64_t)op.left * P.slot_stride + off;
+                {
+                    double a[32];
+                    cols61<0, 32>(Ec, L, a);
+#pragma unroll
+                    for (int x = 0; x < 32; x++) t[x] = a[x] * t[x];
+                }
+                {
+                    double a[29];
+                    cols61<32, 29>(Ec, L, a);
+#pragma unroll
+                    for (int x = 0; x < 29; x++) t[32 + x] = a[x] * t[32 + x];
+                }
+            }
+        }
+    }
+    stage61_uinv(P.inv_evec, E);
+    return mine ? rows61(E, t, dst) : 0.0;
+}
+
 __device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j) {
     const int n = P.n, ncat = P.ncat, block = n * ncat, su = P.su;
     double *M = (double *)P.mats + (int64_t)j.op * P.mat_stride;
@@ -283,8 +427,11 @@ __device__ __forceinline__ double safe_factor(int d) { return d == 0 ? 1.0 : (d 
 // L3 (with MODE_THETA): also reduce the logL at the same length as a third value (the
 // value computeLikelihoodFromBuffer would return there), for the many-branch Newton.
 // Returns true in the workgroup that wrote the result.
-template <int N, int MODE, bool L3 = false>
+// E61: the 61-state ops take op_partial_lds61 (ncat <= 4; the workgroup's LDS then holds
+// the 121 KB matrix buffer).
+template <int N, int MODE, bool L3 = false, bool E61 = false>
 __device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
+    static_assert(!E61 || N == 61, "the LDS matrix path is the 61-state one");
     const int lane = threadIdx.x & 63;
     const int w = wave_id();
     const int ncw = nwaves(), ncat = P.ncat;
@@ -296,6 +443,7 @@ __device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
     __shared__ double s_val[3][PLK_GPU_MAX_CATS * N];
     __shared__ double s_cat[3][PLK_GPU_MAX_CATS][64];
     __shared__ int s_sc[PLK_GPU_MAX_CATS][64];
+    __shared__ __attribute__((aligned(16))) double s_e61[E61 ? kLds61 : 2];
 
     if (MODE != MODE_PARTIALS) fill_val(P, P.br_len, s_val[0], MODE == MODE_THETA ? s_val[1] : nullptr, s_val[2]);
 
@@ -317,7 +465,30 @@ __device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
                 sc_norm = (unsigned)scale[(int64_t)op.left * P.scale_stride + p] + scale[(int64_t)op.right * P.scale_stride + p];
         }
         double wmax = 0.0;
-        for (int c = w; c < ncat; c += ncw) {
+        if constexpr (E61) {
+            // one category per wavefront; the SAFE handling of the N <= 20 path below
+            const int c = w;
+            const bool mine = c < ncat;
+            const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
+            const int64_t a = p * ncat + c;
+            unsigned sc = 0;
+            if (P.safe && mine) {
+                if (op.kind == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + a];
+                else if (op.kind == KIND_II)
+                    sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + a] + scale[(int64_t)op.right * P.scale_stride + a];
+            }
+            const double m = op_partial_lds61(P, op, M, mine, c, off, sl, sr, outp + off, s_e61);
+            if (P.safe && mine) {
+                if (op.kind != KIND_TT && m < kST && invar == 0.0) {
+#pragma unroll
+                    for (int j = 0; j < N; j++) outp[off + j * 64] *= kTwo256;
+                    sc += 1;
+                }
+                scale[(int64_t)op.out_slot * P.scale_stride + a] = (uint16_t)sc;
+            }
+            wmax = m;
+        }
+        for (int c = w; c < (E61 ? 0 : ncat); c += ncw) {
             const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;
             if constexpr (N <= 20) {
                 // SAFE: this category's children counts, also fetched ahead of the work

@@ -21,6 +21,12 @@ __global__ void __launch_bounds__(256, (traverse_min_waves<N, MODE>())) k_traver
     traverse_f64_body<N, MODE>(P, blockIdx.x);
 }
 
+// 61 states with at most four categories: matrices staged in LDS (one workgroup per CU)
+template <int MODE>
+__global__ void __launch_bounds__(256, 1) k_traverse_f64_lds61(KParams P) {
+    traverse_f64_body<61, MODE, false, true>(P, blockIdx.x);
+}
+
 template <int N, int DERIV>
 __global__ void __launch_bounds__(256) k_theta_eval_f64(KParams P) {
     theta_eval_f64_body<N, DERIV>(P, blockIdx.x, P.br_len);
@@ -45,6 +51,12 @@ __global__ void __launch_bounds__(256) k_theta_newton_f64(KParams P, NewtonState
 template <int N>
 hipError_t traverse_n(const KParams &p, int mode, hipStream_t s) {
     dim3 grid(p.ntiles), block(64 * p.ncw);
+    if (N == 61 && p.ncat <= 4) {
+        if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f64_lds61<MODE_PARTIALS>), grid, block, 0, s, p);
+        else if (mode == MODE_LOGL) hipLaunchKernelGGL((k_traverse_f64_lds61<MODE_LOGL>), grid, block, 0, s, p);
+        else hipLaunchKernelGGL((k_traverse_f64_lds61<MODE_THETA>), grid, block, 0, s, p);
+        return hipGetLastError();
+    }
     if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f64<N, MODE_PARTIALS>), grid, block, 0, s, p);
     else if (mode == MODE_LOGL) hipLaunchKernelGGL((k_traverse_f64<N, MODE_LOGL>), grid, block, 0, s, p);
     else hipLaunchKernelGGL((k_traverse_f64<N, MODE_THETA>), grid, block, 0, s, p);

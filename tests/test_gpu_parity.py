@@ -76,6 +76,31 @@ def test_partials_and_scale_counts(safe):
     assert checked > 3
 
 
+@pytest.mark.parametrize("safe,ncat", [(0, 4), (1, 4), (0, 2), (1, 6)])
+def test_codon_partials_and_scale_counts(safe, ncat):
+    """61 states: the LDS-staged form (ncat <= 4) and the scalar-load form (ncat > 4)
+    against the oracle's partials and scale counts.  Both forms run the same sums in the
+    same order (their outputs are identical); eigen-space values of 61 states cancel
+    heavily, so the oracle's host rounding puts ~1e-12 to 1e-11 of the pattern's largest
+    value between it and the device in either form."""
+    rng = np.random.default_rng(9)
+    pp = S.make_problem(61, 16, 300, rng, ncat=ncat, mean_len=0.4, ambiguous=0.05)
+    say("codon partials safe", safe, "ncat", ncat, "nptn", pp.nptn)
+    g = G.Tree(pp, scaling=safe)
+    o = S.Oracle(pp, safe=bool(safe))
+    a, b = 0, int(pp.nei[0, 0])
+    assert rel(g.loglik(a, b), o.loglik(a, b)) <= TOL
+    for v in range(16, 30, 3):
+        dad = int(pp.nei[v, 0])
+        pg, sg = g.partial(dad, v, safe=bool(safe))
+        po, so = o.partial(dad, v, safe=bool(safe))
+        assert np.array_equal(sg, so)
+        scale = np.abs(po).max(axis=(1, 2), keepdims=True)
+        worst = float((np.abs(pg - po) / (scale + 1e-300)).max())
+        say("node", v, "max |diff| / pattern max", worst)
+        assert worst <= 1e-10
+
+
 @pytest.mark.parametrize("safe", [0, 1])
 def test_forced_scaling_caterpillar(safe):
     T = 300
