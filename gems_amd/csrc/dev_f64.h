@@ -182,11 +182,27 @@ __device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j) {
     }
     if (j.leaf) {
         double *T = M + (int64_t)2 * ncat * n * n + (int64_t)j.side * (su + 1) * block;
-        for (int idx = threadIdx.x; idx < su * block; idx += blockDim.x) {
-            int s = idx / block, rem = idx % block, c = rem / n, x = rem % n;
-            double v = 0.0;
-            for (int i = 0; i < n; i++) v = fma(P.evec[x * n + i] * ex[c * n + i], P.tip_lh[s * n + i], v);
-            T[idx] = v;
+        // four entries per thread at a time: independent sums, each in its own order
+        const int nt = blockDim.x, total = su * block;
+        for (int base = threadIdx.x; base < total; base += 4 * nt) {
+            int ss[4], cs[4], xs[4];
+            double v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int idx = min(base + q * nt, total - 1), rem = idx % block;
+                ss[q] = idx / block;
+                cs[q] = rem / n;
+                xs[q] = rem % n;
+                v[q] = 0.0;
+            }
+            for (int i = 0; i < n; i++) {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    v[q] = fma(P.evec[xs[q] * n + i] * ex[cs[q] * n + i], P.tip_lh[ss[q] * n + i], v[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                if (base + q * nt < total) T[base + q * nt] = v[q];
         }
         for (int idx = threadIdx.x; idx < block; idx += blockDim.x) T[(int64_t)su * block + idx] = 1.0;
     }
