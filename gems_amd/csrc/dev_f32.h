@@ -26,7 +26,9 @@ namespace dev {
 // matrix-core forms).
 // Categories c0 .. c1-1 of the job (all by default).
 template <int N>
-__device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j, int c0 = 0, int c1 = -1) {
+// xbo >= 0 (matrix-core forms): only the rows of fragment block xbo -- its Pm rows, its A
+// fragments and its leaf-table rows -- so that a job's blocks run in separate workgroups.
+__device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j, int c0 = 0, int c1 = -1, int xbo = -1) {
     constexpr int XB = f32_xblocks(N), KS = f32_ksteps(N), FR = XB * KS * 64, TROW = f32_trow(N);
     const int ncat = P.ncat, su = P.su;
     if (c1 < 0) c1 = ncat;
@@ -38,31 +40,35 @@ __device__ inline void pmatrix_f32_body(const KParams &P, const PJob &j, int c0 
     for (int c = c0; c < c1; c++) {
         for (int i = threadIdx.x; i < N; i += blockDim.x) ex[i] = exp(P.eval[i] * (P.rates[c] * j.len));
         __syncthreads();
-        for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) {
-            const int x = idx / N, y = idx % N;
+        const int nrow = xbo >= 0 ? 16 : N;
+        for (int idx = threadIdx.x; idx < nrow * N; idx += blockDim.x) {
+            const int x = xbo >= 0 ? f32_orow(N, xbo, idx / N) : idx / N, y = idx % N;
+            if (x < 0) continue;
             double v = 0.0;
             for (int i = 0; i < N; i++) v = fma(P.evec[x * N + i] * ex[i], P.inv_evec[i * N + y], v);
-            Pm[idx] = v;
+            Pm[x * N + y] = v;
         }
         __syncthreads();
         if (N == 4) {
             for (int idx = threadIdx.x; idx < N * N; idx += blockDim.x) F[(int64_t)c * FR + idx] = (float)Pm[idx];
         } else {
-            for (int idx = threadIdx.x; idx < FR; idx += blockDim.x) {
+            const int f0 = xbo >= 0 ? xbo * KS * 64 : 0, f1 = xbo >= 0 ? f0 + KS * 64 : FR;
+            for (int idx = f0 + threadIdx.x; idx < f1; idx += blockDim.x) {
                 const int xb = idx / (KS * 64), ks = (idx / 64) % KS, l = idx % 64;
                 const int x = f32_orow(N, xb, l & 15), y = f32_kstate(N, ks, l >> 4);
                 F[(int64_t)c * FR + idx] = (x >= 0 && y < N) ? (float)Pm[x * N + y] : 0.0f;
             }
         }
         if (j.leaf) {
-            for (int idx = threadIdx.x; idx < su * TROW; idx += blockDim.x) {
-                const int s = idx / TROW, r = idx % TROW, x = f32_orow(N, r / 16, r % 16);
+            const int rw = xbo >= 0 ? 16 : TROW, r0 = xbo >= 0 ? xbo * 16 : 0;
+            for (int idx = threadIdx.x; idx < su * rw; idx += blockDim.x) {
+                const int s = idx / rw, r = r0 + idx % rw, x = f32_orow(N, r / 16, r % 16);
                 double v = 0.0;
                 if (x >= 0)
                     for (int y = 0; y < N; y++) v = fma(Pm[x * N + y], P.tipvec[s * N + y], v);
                 T[((int64_t)s * ncat + c) * TROW + r] = (float)v;
             }
-            for (int r = threadIdx.x; r < TROW; r += blockDim.x)
+            for (int r = r0 + threadIdx.x; r < r0 + rw; r += blockDim.x)
                 T[((int64_t)su * ncat + c) * TROW + r] = f32_orow(N, r / 16, r % 16) >= 0 ? 1.0f : 0.0f;
         }
         __syncthreads();

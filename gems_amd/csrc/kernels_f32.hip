@@ -5,11 +5,13 @@ namespace plk {
 namespace {
 using namespace dev;
 
-// one workgroup per (job, category)
+// one workgroup per (job, category), and for the matrix-core forms per 16-row fragment
+// block of the job's matrices as well
 template <int N>
 __global__ void __launch_bounds__(256) k_pmatrix_f32(KParams P, const PJob *__restrict__ jobs) {
-    const int c = blockIdx.x % P.ncat;
-    pmatrix_f32_body<N>(P, jobs[blockIdx.x / P.ncat], c, c + 1);
+    constexpr int XB = N == 4 ? 1 : f32_xblocks(N);
+    const int xb = blockIdx.x % XB, c = (blockIdx.x / XB) % P.ncat;
+    pmatrix_f32_body<N>(P, jobs[blockIdx.x / (XB * P.ncat)], c, c + 1, N == 4 ? -1 : xb);
 }
 
 // WPC wavefronts per category: 256 threads for up to 4 categories with WPC = 1; WIDE = 1
@@ -129,8 +131,12 @@ hipError_t traverse_f32_n(const KParams &p, int mode, hipStream_t s) {
 hipError_t f32_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s) {
     switch (p.n) {
         case 4: hipLaunchKernelGGL(k_pmatrix_f32<4>, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs); break;
-        case 20: hipLaunchKernelGGL(k_pmatrix_f32<20>, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs); break;
-        case 61: hipLaunchKernelGGL(k_pmatrix_f32<61>, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs); break;
+        case 20:
+            hipLaunchKernelGGL(k_pmatrix_f32<20>, dim3(njobs * p.ncat * f32_xblocks(20)), dim3(256), 0, s, p, jobs);
+            break;
+        case 61:
+            hipLaunchKernelGGL(k_pmatrix_f32<61>, dim3(njobs * p.ncat * f32_xblocks(61)), dim3(256), 0, s, p, jobs);
+            break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
