@@ -164,34 +164,37 @@ __device__ __forceinline__ double op_partial_lds61(const KParams &P, const DevOp
     return mine ? rows61(E, t, dst) : 0.0;
 }
 
-__device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j) {
+// cat >= 0: category cat only (the job's categories then run in separate workgroups).
+__device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j, const int cat = -1) {
     const int n = P.n, ncat = P.ncat, block = n * ncat, su = P.su;
+    const int cb = cat >= 0 ? cat : 0, nc = cat >= 0 ? 1 : ncat;  // categories cb .. cb+nc-1
     double *M = (double *)P.mats + (int64_t)j.op * P.mat_stride;
     double *E = M + (int64_t)j.side * ncat * n * n;
     __shared__ double ex[PLK_GPU_MAX_CATS * PLK_MAX_STATES];
-    for (int idx = threadIdx.x; idx < ncat * n; idx += blockDim.x) {
-        int c = idx / n, i = idx % n;
-        ex[idx] = exp(P.eval[i] * (P.rates[c] * j.len));
+    for (int idx = threadIdx.x; idx < nc * n; idx += blockDim.x) {
+        const int c = cb + idx / n, i = idx % n;
+        ex[c * n + i] = exp(P.eval[i] * (P.rates[c] * j.len));
     }
     __syncthreads();
     // stored [c][i][x] (a column per i) for n <= 20, whose kernels read a column at a time
     const bool cols = n <= 20;
-    for (int idx = threadIdx.x; idx < ncat * n * n; idx += blockDim.x) {
-        int c = idx / (n * n), rem = idx % (n * n), x = rem / n, i = rem % n;
-        E[cols ? (int64_t)c * n * n + i * n + x : idx] = P.evec[x * n + i] * ex[c * n + i];
+    for (int idx = threadIdx.x; idx < nc * n * n; idx += blockDim.x) {
+        const int c = cb + idx / (n * n), rem = idx % (n * n), x = rem / n, i = rem % n;
+        E[(int64_t)c * n * n + (cols ? i * n + x : rem)] = P.evec[x * n + i] * ex[c * n + i];
     }
     if (j.leaf) {
         double *T = M + (int64_t)2 * ncat * n * n + (int64_t)j.side * (su + 1) * block;
-        // four entries per thread at a time: independent sums, each in its own order
-        const int nt = blockDim.x, total = su * block;
+        // leaf[s][c][x]; four entries per thread at a time: independent sums, each in its
+        // own order
+        const int nt = blockDim.x, per_s = nc * n, total = su * per_s;
         for (int base = threadIdx.x; base < total; base += 4 * nt) {
             int ss[4], cs[4], xs[4];
             double v[4];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const int idx = min(base + q * nt, total - 1), rem = idx % block;
-                ss[q] = idx / block;
-                cs[q] = rem / n;
+                const int idx = min(base + q * nt, total - 1), rem = idx % per_s;
+                ss[q] = idx / per_s;
+                cs[q] = cb + rem / n;
                 xs[q] = rem % n;
                 v[q] = 0.0;
             }
@@ -202,9 +205,9 @@ __device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j) {
             }
 #pragma unroll
             for (int q = 0; q < 4; q++)
-                if (base + q * nt < total) T[base + q * nt] = v[q];
+                if (base + q * nt < total) T[(int64_t)ss[q] * block + cs[q] * n + xs[q]] = v[q];
         }
-        for (int idx = threadIdx.x; idx < block; idx += blockDim.x) T[(int64_t)su * block + idx] = 1.0;
+        for (int idx = threadIdx.x; idx < nc * n; idx += blockDim.x) T[(int64_t)su * block + cb * n + idx] = 1.0;
     }
 }
 

@@ -5,8 +5,9 @@ namespace plk {
 namespace {
 using namespace dev;
 
+// one workgroup per (job, category)
 __global__ void __launch_bounds__(256) k_pmatrix_f64(KParams P, const PJob *__restrict__ jobs) {
-    pmatrix_f64_body(P, jobs[blockIdx.x]);
+    pmatrix_f64_body(P, jobs[blockIdx.x / P.ncat], blockIdx.x % P.ncat);
 }
 
 // Occupancy floor: the 20-state partial/logL kernels are held to 128 VGPRs (4 waves per
@@ -94,7 +95,7 @@ hipError_t f64_theta_newton(const KParams &p, NewtonState *st, hipStream_t s) {
 }
 
 hipError_t f64_pmatrix(const KParams &p, const PJob *jobs, int njobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_pmatrix_f64, dim3(njobs), dim3(256), 0, s, p, jobs);
+    hipLaunchKernelGGL(k_pmatrix_f64, dim3(njobs * p.ncat), dim3(256), 0, s, p, jobs);
     return hipGetLastError();
 }
 
