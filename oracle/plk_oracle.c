@@ -370,7 +370,13 @@ static void partial_slice(orc_tree *t, const orc_op *op, size_t lo, size_t hi) {
                 }
         }
         for (int c = 0; c < ncat; c++) {
+#ifdef ORC_PLANT_CATEGORY_MAX
+            /* planted mistake, built only by tests/test_oracle.py: NORM keeps a
+             * per-category maximum, so its check sees the last category alone */
+            if (1)
+#else
             if (safe)
+#endif
                 for (int v = 0; v < V; v++) lh_max[v] = 0.0;
             const double *el = op->eleft + (size_t)c * n * n;
             const double *er = op->eright + (size_t)c * n * n;

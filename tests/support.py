@@ -42,7 +42,18 @@ def oracle_lib(lanes=None):
     path = ORACLE_DIR / f"libplk_oracle_v{lanes}.so"
     if not path.exists():
         raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
-    lib = ctypes.CDLL(str(path))
+    lib = _bind(ctypes.CDLL(str(path)))
+    if _oracle is None or lanes is None:
+        _oracle = lib
+    return lib
+
+
+def load_oracle(path):
+    """An oracle library built elsewhere (the planted-mistake build of a test)."""
+    return _bind(ctypes.CDLL(str(path)))
+
+
+def _bind(lib):
     P = ctypes.c_void_p
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
@@ -92,8 +103,6 @@ def oracle_lib(lanes=None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if _oracle is None or lanes is None:
-        _oracle = lib
     return lib
 
 
@@ -104,8 +113,8 @@ def _dp(a):
 class Oracle:
     """Thin object wrapper over one orc_tree."""
 
-    def __init__(self, prob, safe=False, nthreads=2, lanes=None):
-        self.lib = oracle_lib(lanes)
+    def __init__(self, prob, safe=False, nthreads=2, lanes=None, lib=None):
+        self.lib = lib if lib is not None else oracle_lib(lanes)
         self.p = prob
         n, ncat = prob.n, len(prob.rates)
         self.h = self.lib.orc_create(n, ncat, prob.ntaxa, prob.nptn, int(safe), nthreads)
@@ -584,3 +593,113 @@ class OracleRell:
                                _dp(ptn_lh), cur_logl, eps, u, _dp(rell), _dp(self.logl),
                                self.best_counts.ctypes.data_as(i32), _dp(self.orig), improved.ctypes.data_as(i32))
         return rell, improved
+
+
+# ---------------------------------------------------------------------------------------
+# scale_num counts derived from the reference's rule, independently of the oracle
+
+
+def derived_scale_counts(prob, dad, node, safe, rule="reference", memo=None):
+    """Scale counts of node's subtree seen from dad, derived from the reference's rule
+    (tree/phylokernelnew.h:1643-1644 tip-tip: 0 and no check; 1757-1760 tip-internal: the
+    internal child's count; 1929-1936 internal-internal: the sum; then 1997-2034 /
+    1893-1907: one increment when the largest |entry| of the eigen-space partial, over
+    the whole pattern block (NORM) or over one category (SAFE), is below 2^-256 and
+    ptn_invar is 0).
+
+    Nothing here shares the oracle's arithmetic.  Each direction carries the real-space
+    conditional likelihood as a mantissa per (category, pattern, state) times 2^E with an
+    integer exponent E per (category, pattern), propagated with scipy expm and
+    renormalised with frexp after every node, so the true values never underflow.  The
+    eigen-space partial the rule inspects is U^-1 applied to that vector
+    (tree/phylokernelnew.h:1989-1994), and the value the kernel checks is the true one
+    times 2^(256 * children's counts), all scalings being exact powers of two.  So the
+    count follows from log2 of the true maximum alone.
+
+    rule selects planted variants used to show the check is sensitive:
+      "reference"        the rule above
+      "last_category"    NORM check on the last category's maximum only (a per-category
+                         maximum left over from SAFE)
+      "threshold_2^-255" a threshold one binade too high
+      "before_projection" the maximum taken over the real-space product, before U^-1
+
+    Returns (counts, ambiguous): counts [ptn] (NORM) or [ptn, ncat] (SAFE) as int64, and
+    a boolean mask of patterns whose checked maximum lies within 1e-6 (log2 units) of the
+    threshold, where the rounding of either computation could decide the count.
+    """
+    if memo is None:
+        memo = {}
+    nei, blen = prob.nei, prob.blen
+    ncat, n, nptn = len(prob.rates), prob.n, prob.nptn
+    invar = np.zeros(nptn) if getattr(prob, "invar", None) is None else np.asarray(prob.invar)
+    thr = -255.0 if rule == "threshold_2^-255" else -256.0
+    pcache = memo.setdefault("_P", {})
+    done = memo.setdefault((bool(safe), rule), {})
+
+    def pmat(t, c):
+        key = (float(t), c)
+        if key not in pcache:
+            pcache[key] = scipy.linalg.expm(prob.Q * prob.rates[c] * t)
+        return pcache[key]
+
+    def tips(node_):
+        T = np.stack([tip_likelihood(prob, s) for s in range(prob.su + 1)])
+        return np.repeat(T[prob.states[node_]][None], ncat, axis=0)  # c x ptn x n
+
+    def walk(d, v):
+        # returns (mantissa c x ptn x n, exponent c x ptn, counts, ambiguous, is_tip)
+        key = (d, v)
+        if key in done:
+            return done[key]
+        if v < prob.ntaxa:
+            res = (tips(v), np.zeros((ncat, nptn), dtype=np.int64), None, np.zeros(nptn, bool), True)
+            done[key] = res
+            return res
+        kids = [(int(nei[v, k]), blen[v, k]) for k in range(3) if nei[v, k] >= 0 and nei[v, k] != d]
+        assert len(kids) == 2
+        sub = [walk(v, ch) for (ch, _t) in kids]
+        mant = np.ones((ncat, nptn, n))
+        expo = np.zeros((ncat, nptn), dtype=np.int64)
+        for (ch, t), (m, e, _s, _a, _tip) in zip(kids, sub):
+            for c in range(ncat):
+                mant[c] *= m[c] @ pmat(t, c).T
+            expo += e
+        # renormalise: mantissa maximum in [0.5, 1)
+        mx = np.abs(mant).max(axis=2)
+        _f, ex = np.frexp(np.where(mx > 0, mx, 1.0))
+        mant = mant / np.ldexp(1.0, ex)[..., None]
+        expo = expo + ex
+        eig = np.einsum("jx,cpx->cpj", prob.inv_evec, mant)
+        src = mant if rule == "before_projection" else eig
+        with np.errstate(divide="ignore"):
+            lmax = np.log2(np.abs(src).max(axis=2)) + expo  # c x ptn, log2 of the true maximum
+        tip_flags = [s[4] for s in sub]
+        child_counts = [s[2] for s in sub if not s[4]]
+        shape = (nptn, ncat) if safe else (nptn,)
+        base = np.zeros(shape, dtype=np.int64)
+        for cc in child_counts:
+            base = base + cc
+        amb = np.zeros(nptn, bool)
+        for s in sub:
+            amb |= s[3]
+        if all(tip_flags):
+            counts = base  # tip-tip: 0, no check
+        elif safe:
+            pre = lmax.T + 256.0 * base  # ptn x c
+            hit = (pre < thr) & (invar[:, None] == 0.0)
+            amb |= np.any(np.abs(pre - thr) < 1e-6, axis=1)
+            counts = base + hit
+        else:
+            m = lmax[-1] if rule == "last_category" else lmax.max(axis=0)
+            pre = m + 256.0 * base
+            hit = (pre < thr) & (invar == 0.0)
+            amb |= np.abs(pre - thr) < 1e-6
+            counts = base + hit
+        res = (mant, expo, counts, amb, False)
+        done[key] = res
+        return res
+
+    _m, _e, counts, amb, is_tip = walk(dad, node)
+    if is_tip:
+        raise ValueError("a tip has no scale counts")
+    return counts, amb

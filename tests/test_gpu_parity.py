@@ -276,3 +276,33 @@ def test_device_newton_abi_errors():
     assert rc == -4 and b"Maximum number of iterations" in L.plk_last_error()
     rc = L.plk_newton_branch(ctx, ra, rb, 1e-6, 0.1, 10.0, 1e-6, 100, ctypes.byref(out), ctypes.byref(ev))
     assert rc == 0 and 1e-6 <= out.value <= 10.0 and ev.value >= 1
+
+
+SCALE_RULE_CASES = [(4, 300, 150, 1.5, 0), (4, 300, 150, 1.5, 1), (20, 100, 120, 1.5, 0), (20, 100, 120, 1.5, 1),
+                    (61, 40, 60, 1.0, 1)]
+
+
+@pytest.mark.parametrize("case", SCALE_RULE_CASES, ids=lambda c: f"n{c[0]}-{'safe' if c[4] else 'norm'}")
+def test_scale_counts_follow_rule(case):
+    """The device's scale counts on internal directions of a deep caterpillar equal the
+    counts derived from the reference's rule (tree/phylokernelnew.h:1643-1644,
+    1757-1760, 1929-1936, 1997-2034) by support.derived_scale_counts, which tracks the
+    true magnitudes with explicit exponents and shares no arithmetic with the oracle."""
+    n, T, sites, blen, safe = case
+    rng = np.random.default_rng(5)
+    pp = S.make_problem(n, T, sites, rng, tree=S.caterpillar_tree(T, blen), ambiguous=0.05)
+    say("scale rule", case, "nptn", pp.nptn)
+    g = G.Tree(pp, scaling=safe)
+    memo = {}
+    checked = top = 0
+    for v in list(range(T, 2 * T - 2, 7)) + [2 * T - 3]:
+        for k in range(3):
+            d = int(pp.nei[v, k])
+            _, sg = g.partial(d, v, safe=bool(safe))
+            sg = sg.reshape(pp.nptn, -1).astype(np.int64) if safe else sg.astype(np.int64)
+            dc, amb = S.derived_scale_counts(pp, d, v, bool(safe), memo=memo)
+            assert amb.sum() <= 1
+            assert np.array_equal(sg[~amb], dc[~amb]), (v, d)
+            top = max(top, int(sg.max()))
+            checked += 1
+    assert checked >= 3 * (T - 2) // 7 and top >= 2

@@ -258,3 +258,111 @@ def test_nni_eval_equals_swapped_tree_from_scratch():
                 checked += 1
     assert checked >= 8
     assert o.loglik(a, b) == before
+
+
+def _scale_cases():
+    # (states, taxa, sites, caterpillar branch length, safe)
+    return [(4, 300, 150, 1.5, False), (4, 300, 150, 1.5, True), (20, 100, 120, 1.5, False),
+            (20, 100, 120, 1.5, True), (61, 40, 60, 1.0, True)]
+
+
+def _scale_problem(n, T, sites, blen, seed=5):
+    rng = np.random.default_rng(seed)
+    return S.make_problem(n, T, sites, rng, tree=S.caterpillar_tree(T, blen), ambiguous=0.05)
+
+
+def _count_mismatches(o, p, safe, memo, rule="reference"):
+    """Compare o's scale counts with the rule-derived ones on every internal direction."""
+    T = p.ntaxa
+    o.loglik(0, int(p.nei[0, 0]))
+    bad = total = amb_total = top = 0
+    for v in range(T, 2 * T - 2):
+        for k in range(3):
+            d = int(p.nei[v, k])
+            _, so = o.partial(d, v, safe=safe)
+            so = so.reshape(p.nptn, -1).astype(np.int64) if safe else so.astype(np.int64)
+            dc, amb = S.derived_scale_counts(p, d, v, safe, rule=rule, memo=memo)
+            bad += int(np.sum((so != dc)[~amb]))
+            amb_total += int(amb.sum())
+            total += so.size
+            top = max(top, int(so.max()))
+    return bad, total, amb_total, top
+
+
+@pytest.mark.parametrize("case", _scale_cases(), ids=lambda c: f"n{c[0]}-{'safe' if c[4] else 'norm'}")
+def test_scale_counts_follow_rule(case):
+    """Every scale count the oracle stores, on every direction of a deep caterpillar (many
+    forced 2^-256 rescalings), equals the count derived from the reference's rule by
+    support.derived_scale_counts, which tracks the true magnitudes with explicit
+    exponents and shares no arithmetic with the oracle.  The planted variants of the
+    rule (NORM maximum of the last category only, a threshold of 2^-255, the maximum
+    taken before the U^-1 projection) each give different counts on the same problem,
+    so a mistake of any of these kinds in the oracle would fail this test."""
+    n, T, sites, blen, safe = case
+    p = _scale_problem(n, T, sites, blen)
+    o = S.Oracle(p, safe=safe)
+    memo = {}
+    bad, total, amb, top = _count_mismatches(o, p, safe, memo)
+    assert total > 0 and amb <= total * 1e-4
+    assert bad == 0
+    assert top >= 2  # rescaling happened more than once along a path
+    for rule in ("last_category", "threshold_2^-255", "before_projection"):
+        if rule == "last_category" and safe:
+            continue
+        diff = 0
+        for v in range(T, 2 * T - 2):
+            for k in range(3):
+                d = int(p.nei[v, k])
+                ref, _ = S.derived_scale_counts(p, d, v, safe, memo=memo)
+                var, _ = S.derived_scale_counts(p, d, v, safe, rule=rule, memo=memo)
+                diff += int(np.sum(ref != var))
+        assert diff > 0, rule
+
+
+def test_scale_counts_tip_cases():
+    """Tip-tip directions store 0 (no check); a tip-internal direction stores the
+    internal child's count plus at most one (tree/phylokernelnew.h:1643-1644, 1757-1760);
+    internal-internal the children's sum plus at most one (1929-1936)."""
+    p = _scale_problem(4, 120, 100, 1.5, seed=8)
+    for safe in (False, True):
+        o = S.Oracle(p, safe=safe)
+        o.loglik(0, int(p.nei[0, 0]))
+        T = p.ntaxa
+        for v in range(T, 2 * T - 2):
+            for k in range(3):
+                d = int(p.nei[v, k])
+                kids = [int(x) for x in p.nei[v] if x >= 0 and x != d]
+                _, s = o.partial(d, v, safe=safe)
+                s = s.astype(np.int64)
+                base = np.zeros_like(s)
+                for ch in kids:
+                    if ch >= T:
+                        _, sc = o.partial(v, ch, safe=safe)
+                        base += sc.astype(np.int64)
+                if all(ch < T for ch in kids):
+                    assert not s.any()
+                else:
+                    inc = s - base
+                    assert inc.min() >= 0 and inc.max() <= 1
+
+
+def test_planted_category_max_in_oracle_is_caught(tmp_path):
+    """The oracle rebuilt with a planted mistake (NORM's maximum reset per category, so
+    its check sees the last category only) fails the rule comparison."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    so = tmp_path / "libplk_oracle_planted.so"
+    subprocess.run([cc, "-O2", "-fPIC", "-fopenmp", "-std=c11", "-DORC_V=4", "-DORC_PLANT_CATEGORY_MAX",
+                    "-march=x86-64-v3", "-shared", "-o", str(so), str(S.ORACLE_DIR / "plk_oracle.c"), "-lm"],
+                   check=True)
+    lib = S.load_oracle(so)
+    p = _scale_problem(4, 300, 150, 1.5)
+    bad, total, _, _ = _count_mismatches(S.Oracle(p, safe=False, lib=lib), p, False, {})
+    assert bad > 0
+    # the logL is blind to where the power-of-two scaling happens
+    good = S.Oracle(p, safe=False).loglik(0, int(p.nei[0, 0]))
+    planted = S.Oracle(p, safe=False, lib=lib).loglik(0, int(p.nei[0, 0]))
+    assert abs(good - planted) <= 1e-11 * abs(good)
