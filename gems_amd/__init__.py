@@ -106,6 +106,7 @@ _SIGS = {
     "plk_gamma_rates": (_int, [_dbl, _int, _dp]),
     "plk_gamma_rates_median": (_int, [_dbl, _int, _dp]),
     "plk_tip_lh": (_int, [_int, _int, _dp, _dp]),
+    "plk_ptn_invar": (_int, [_int, _int, _int, _int, _u8p, _dp, _dbl, _dp]),
     "plk_codon_exch": (_int, [_dbl, _dbl, _dp]),
     "plk_sim_create": (_int, [ctypes.POINTER(_P), _int, _int, _int, _dbl, _dbl, _int, ctypes.c_uint64]),
     "plk_sim_destroy": (_int, [_P]),
@@ -211,6 +212,16 @@ def probe_copy(nbytes=1 << 30, reps=5):
     v = ctypes.c_double()
     _check(lib().plk_probe_copy(nbytes, reps, ctypes.byref(v)))
     return v.value
+
+
+def ptn_invar(n, seq_type, states, pi, p_invar):
+    """plk_ptn_invar: invariant-site mass per pattern for proportion p_invar."""
+    st = np.ascontiguousarray(states, dtype=np.uint8)
+    pi = np.ascontiguousarray(pi, dtype=np.float64)
+    out = np.zeros(st.shape[1])
+    _check(lib().plk_ptn_invar(n, seq_type, st.shape[0], st.shape[1], st.ctypes.data_as(_u8p), _d(pi), p_invar,
+                               _d(out)))
+    return out
 
 
 class Problem:

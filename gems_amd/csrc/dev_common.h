@@ -17,6 +17,19 @@ constexpr double kLn2 = 0.69314718055994530941723212145818;
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// log(|s| 2^e + invar): a pattern likelihood held as a mantissa s and a power-of-two exponent
+// e (the FP32 form renormalises every pattern) plus the invariant-site mass, which is a true
+// (unscaled) value.  The reference adds ptn_invar to an unscaled likelihood
+// (tree/phylokernelnew.h:3097) because it never rescales a pattern with ptn_invar != 0
+// (:1999, :2020); here the sum is formed in log space, so no 2^e or 2^-e is ever formed.
+__device__ inline double log_lh_expo(double s, int e, double invar) {
+    const double ls = log(fabs(s)) + e * kLn2;
+    if (invar == 0.0) return ls;
+    const double li = log(invar);
+    const double hi = fmax(ls, li), lo = fmin(ls, li);
+    return hi + log1p(exp(lo - hi));
+}
+
 // Read-only views in the constant address space: wave-uniform reads through them
 // compile to scalar loads whose results feed the FMAs as SGPR operands (the data is
 // written by an earlier launch only).

@@ -457,9 +457,8 @@ __device__ inline void traverse_f32_mfma_body(const KParams &P, const int tile, 
         const bool mine = p < P.nptn && (blk < 0 || (lane >> 4) == blk);  // this workgroup's patterns
         double lh = 0.0;
         for (int cc = 0; cc < ncat; cc++) lh += P.props[cc] * s_cat[cc][lane];
-        lh = fabs(lh) + P.invar[p];
         const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
-        const double l = log(lh) + e * kLn2;
+        const double l = log_lh_expo(lh, e, P.invar[p]);
         if (mine) P.ptn_lh[p] = l;
         r0 = wave_sum(mine ? l * P.freq[p] : 0.0);
     }
@@ -622,9 +621,8 @@ __device__ inline void traverse_f32_pbw_body(const KParams &P, const int tile) {
     }
     double contrib = 0.0;
     if (active && lg == 0) {
-        lh = fabs(lh) + P.invar[p];
         const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
-        const double l = log(lh) + e * kLn2;
+        const double l = log_lh_expo(lh, e, P.invar[p]);
         if (p < P.nptn) {
             P.ptn_lh[p] = l;
             contrib = l * P.freq[p];
@@ -756,9 +754,8 @@ __device__ inline void traverse_f32_valu_body(const KParams &P, const int tile) 
     if (w == 0) {
         double lh = 0.0;
         for (int cc = 0; cc < ncat; cc++) lh += P.props[cc] * s_cat[cc][lane];
-        lh = fabs(lh) + P.invar[p];
         const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
-        const double l = log(lh) + e * kLn2;
+        const double l = log_lh_expo(lh, e, P.invar[p]);
         if (p < P.nptn) P.ptn_lh[p] = l;
         r0 = wave_sum(p < P.nptn ? l * P.freq[p] : 0.0);
     }
@@ -795,6 +792,14 @@ __device__ inline bool theta_f32_body(const KParams &P, const int tile) {
         s_val[2][idx] = cof * val1;
     }
     __syncthreads();
+    // A pattern with invariant-site mass gets theta in the true (unscaled) frame and a zero
+    // buffer_scale_all, as the reference's never-rescaled +I patterns have
+    // (tree/phylokernelnew.h:1999, 2020), so every reader adds ptn_invar to a true value.
+    // ldexp may take such a theta into the subnormals or to zero; the invariant mass then
+    // dominates that pattern's likelihood.
+    const int *expo = (const int *)P.scale;
+    const int e_all = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
+    const bool has_inv = P.invar[p] != 0.0;
     for (int c = w; c < ncat; c += ncw) {
         const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + lane;  // theta (FP64 layout)
         const int64_t offf = ((int64_t)tile * ncat + c) * N * 64 + f32_loff(N, lane);
@@ -814,7 +819,7 @@ __device__ inline bool theta_f32_body(const KParams &P, const int tile) {
                 ai = fma(piU[i * N + x], a[x], ai);
                 bi = fma(uinv[i * N + x], b[x], bi);
             }
-            const double th = ai * bi;
+            const double th = has_inv ? ldexp(ai * bi, e_all) : ai * bi;
             if (theta) theta[off + (int64_t)i * 64] = th;  // null: derivatives only
             l0 = fma(s_val[0][c * N + i], th, l0);
             l1 = fma(s_val[1][c * N + i], th, l1);
@@ -827,8 +832,7 @@ __device__ inline bool theta_f32_body(const KParams &P, const int tile) {
     __syncthreads();
     double r0 = 0.0, r1 = 0.0, r2 = 0.0;
     if (w == 0) {
-        const int *expo = (const int *)P.scale;
-        const int e = expo[(int64_t)P.br_b * P.scale_stride + p] + (atip ? 0 : expo[(int64_t)P.br_a * P.scale_stride + p]);
+        const int e = has_inv ? 0 : e_all;
         if (P.buf_scale) P.buf_scale[p] = e * kLn2;
         double lh = 0.0, d1 = 0.0, d2 = 0.0;
         for (int c = 0; c < ncat; c++) {

@@ -451,6 +451,7 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
     double r0 = 0.0;
     if (w == 0) {
         double lh = 0.0, sc;
+        int se = 0;  // FP32 form: the pattern's power-of-two exponent
         if (FORM == 0 && P.safe) {
             // 3063-3088: categories above the minimum count are scaled by 2^-256 or dropped
             int mn = s_sc[0][lane];
@@ -464,10 +465,12 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
             for (int c = 0; c < ncat; c++) lh += FORM == 0 ? s_cat[c][lane] : P.props[c] * s_cat[c][lane];
             const int s = small_scale<REAL>(P, L, P.br_b, P.br_lds_b, 0, lane, p) +
                           (atip ? 0 : small_scale<REAL>(P, L, P.br_a, P.br_lds_a, 0, lane, p));
-            sc = FORM == 0 ? (double)(unsigned)s * kLogST : (double)s * kLn2;
+            sc = FORM == 0 ? (double)(unsigned)s * kLogST : 0.0;
+            se = s;
         }
-        lh = fabs(lh) + invar;
-        const double l = log(lh) + sc;
+        // FP64 counts are 0 on patterns with invariant mass (never rescaled), so the sum is
+        // unscaled there; the FP32 exponent is folded in by log_lh_expo.
+        const double l = FORM == 0 ? log(fabs(lh) + invar) + sc : log_lh_expo(lh, se, invar);
         if (p < P.nptn) P.ptn_lh[p] = l;
         r0 = wave_sum(p < P.nptn ? l * P.freq[p] : 0.0);
     }

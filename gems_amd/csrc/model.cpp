@@ -302,6 +302,52 @@ extern "C" int plk_tip_lh(int n, int seq_type, const double *inv_evec, double *t
     return PLK_OK;
 }
 
+// Invariant-site mass per pattern.  Alignment::computeConst (alignment/alignment.cpp:1321-1420)
+// intersects, over the taxa, the set of states each observed state allows
+// (Alignment::getAppearance, :5567-5612) and codes the result as const_char: a single state,
+// STATE_UNKNOWN when every state is allowed, a DNA ambiguity code, protein B/Z/J, or
+// STATE_UNKNOWN + 1 when the intersection is empty.  PhyloTree::computePtnInvar
+// (tree/phylotreesse.cpp:549-612) then sets p_invar times the frequency of that state (times
+// the sum of the allowed states' frequencies for an ambiguity code, 1 for STATE_UNKNOWN) and 0
+// for non-constant patterns.
+extern "C" int plk_ptn_invar(int n, int seq_type, int ntaxa, int nptn, const uint8_t *states, const double *pi,
+                             double p_invar, double *ptn_invar) {
+    if (n < 2 || n > PLK_MAX_STATES || ntaxa < 1 || nptn < 0 || (nptn > 0 && (!states || !ptn_invar)) || !pi ||
+        !(p_invar >= 0.0 && p_invar < 1.0))
+        return plk::fail(PLK_ERR_ARG, "plk_ptn_invar: bad arguments");
+    const int su = plk_state_unknown(n, seq_type);
+    // allowed-state sets as bit masks over up to 64 states (61 codons fit one word)
+    std::vector<uint64_t> allow(256, 0);
+    const uint64_t all = (n == 64) ? ~0ull : ((1ull << n) - 1);
+    for (int s = 0; s < 256; s++) {
+        if (s == su) allow[s] = all;
+        else if (s < n) allow[s] = 1ull << s;
+        else if (seq_type == PLK_SEQ_DNA && s < su) allow[s] = (uint64_t)(s - (n - 1)) & all;
+        else if (seq_type == PLK_SEQ_PROTEIN && s < n + 3) {
+            static const int ambi[6] = {2, 3, 5, 6, 9, 10};  // B = N|D, Z = Q|E, J = I|L
+            allow[s] = (1ull << ambi[(s - n) * 2]) | (1ull << ambi[(s - n) * 2 + 1]);
+        } else allow[s] = ~0ull;  // not a valid state code: checked below
+    }
+    for (int p = 0; p < nptn; p++) {
+        uint64_t m = all;
+        for (int t = 0; t < ntaxa; t++) {
+            const uint8_t s = states[(size_t)t * nptn + p];
+            if (s > su) return plk::fail(PLK_ERR_ARG, "plk_ptn_invar: state code above STATE_UNKNOWN");
+            m &= allow[s];
+        }
+        double v = 0.0;
+        if (m == all) {
+            v = p_invar;  // const_char == STATE_UNKNOWN
+        } else if (m != 0) {
+            for (int x = 0; x < n; x++)
+                if (m & (1ull << x)) v += pi[x];
+            v *= p_invar;
+        }
+        ptn_invar[p] = v;
+    }
+    return PLK_OK;
+}
+
 // Standard genetic code over codons AAA..TTT (A,C,G,T order), '*' = stop
 // (alignment/alignment.cpp:44 uses the same NCBI table 1 in this order).
 namespace plk {

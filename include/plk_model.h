@@ -50,6 +50,15 @@ int plk_gamma_rates_median(double alpha, int ncat, double *rates);
 /* tip_partial_lh[(su+1) x n] = Uinv * tip likelihood of each state */
 int plk_tip_lh(int n, int seq_type, const double *inv_evec, double *tip_lh);
 
+/* ptn_invar for a +I model (replaces PhyloTree::computePtnInvar, tree/phylotreesse.cpp:549-612,
+ * with the constant-state code of Alignment::computeConst, alignment/alignment.cpp:1321-1420):
+ * ptn_invar[p] = p_invar * (sum of pi over the states every taxon of pattern p allows),
+ * 0 for patterns with no common state.  states is ntaxa x nptn (plk_set_patterns' layout).
+ * The matching rate model has props (1 - p_invar)/ncat and Gamma rates / (1 - p_invar)
+ * (model/rategammainvar.h:69, model/rategamma.cpp:129-140). */
+int plk_ptn_invar(int n, int seq_type, int ntaxa, int nptn, const uint8_t *states, const double *pi,
+                  double p_invar, double *ptn_invar);
+
 /* Exchangeabilities of the 61-state GY codon model of the standard code for (kappa,
  * omega); sense codons in AAA..TTT order. */
 int plk_codon_exch(double kappa, double omega, double *exch);

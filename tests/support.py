@@ -529,7 +529,61 @@ def pruning_loglik(prob, a, b, dtype=np.float64):
     for c in range(ncat):
         lh += prob.props[c] * np.einsum("px,px->p", La[c] * prob.pi[None, :], Lb[c] @ pmat(t, c).T)
     lp = np.log(lh) + sa + sb
+    inv = getattr(prob, "invar", None)
+    if inv is not None:
+        # + invariant-site mass, a true (unscaled) value: log(lh e^(sa+sb) + invar)
+        with np.errstate(divide="ignore"):
+            lp = np.logaddexp(lp, np.log(np.asarray(inv)))
     return lp, float(np.dot(lp, prob.freq))
+
+
+def ptn_invar_identity(prob, p_inv):
+    """Invariant-site mass of each pattern from its definition: p_inv times the probability
+    that every taxon shows one common unchanged state, i.e. p_inv * sum_x pi_x * prod_t
+    [state of taxon t allows x].  Independent of the product's const_char coding."""
+    A = np.stack([tip_likelihood(prob, s) for s in range(prob.su + 1)])  # allowed-state indicators
+    ind = np.prod(A[np.asarray(prob.states)], axis=0)  # nptn x n
+    return p_inv * (ind @ np.asarray(prob.pi))
+
+
+def apply_pinvar(prob, p_inv):
+    """+I+G on a problem: props (1 - p)/ncat, Gamma rates / (1 - p)
+    (model/rategammainvar.h:69, model/rategamma.cpp:129-140) and ptn_invar."""
+    ncat = len(prob.rates)
+    prob.props = np.full(ncat, (1.0 - p_inv) / ncat)
+    prob.rates = np.ascontiguousarray(np.asarray(prob.rates) / (1.0 - p_inv))
+    prob.invar = ptn_invar_identity(prob, p_inv)
+    return prob
+
+
+def plant_constant_columns(prob, frac, rng):
+    """Overwrite a fraction of the pattern columns with constant ones (one state per column,
+    some taxa then shown as an ambiguity code allowing it, or unknown); keeps the pattern
+    count and frequencies (duplicated columns stay separate patterns, which the likelihood
+    does not care about)."""
+    st = np.array(prob.states, dtype=np.uint8)
+    T, nptn = st.shape
+    cols = rng.choice(nptn, size=max(1, int(frac * nptn)), replace=False)
+    for j in cols:
+        x = int(rng.integers(prob.n))
+        col = np.full(T, x, dtype=np.uint8)
+        k = int(rng.integers(0, 3))
+        who = rng.choice(T, size=k, replace=False)
+        for t in who:
+            if prob.seq_type == SEQ_DNA:
+                # an ambiguity code whose mask includes x, or unknown
+                masks = [m for m in range(1, 16) if m & (1 << x) and bin(m).count("1") >= 2]
+                m = int(rng.choice(masks))
+                col[t] = m + 3
+            elif prob.seq_type == SEQ_PROTEIN and x in (2, 3, 5, 6, 9, 10):
+                col[t] = 20 + [(2, 3), (5, 6), (9, 10)].index(next(pr for pr in [(2, 3), (5, 6), (9, 10)] if x in pr))
+            else:
+                col[t] = prob.su
+        st[:, j] = col
+    # one all-unknown column
+    st[:, cols[0]] = prob.su
+    prob.states = np.ascontiguousarray(st)
+    return prob
 
 
 def brute_force_loglik(prob, a, b):
@@ -568,6 +622,8 @@ def brute_force_loglik(prob, a, b):
                     term = term * (tips[v] @ M[st[u]])
             lc += term
         total += prob.props[c] * lc
+    if getattr(prob, "invar", None) is not None:
+        total = total + np.asarray(prob.invar)
     lp = np.log(total)
     return lp, float(np.dot(lp, prob.freq))
 

@@ -366,3 +366,82 @@ def test_planted_category_max_in_oracle_is_caught(tmp_path):
     good = S.Oracle(p, safe=False).loglik(0, int(p.nei[0, 0]))
     planted = S.Oracle(p, safe=False, lib=lib).loglik(0, int(p.nei[0, 0]))
     assert abs(good - planted) <= 1e-11 * abs(good)
+
+
+# ---------------------------------------------------------------------------------------
+# +I: invariant-site mass (ptn_invar)
+
+
+def _invar_problem(n, T, sites, seed, tree=None, p_inv=0.3, frac=0.3, ambiguous=0.05):
+    rng = np.random.default_rng(seed)
+    p = S.make_problem(n, T, sites, rng, tree=tree, ambiguous=ambiguous)
+    S.plant_constant_columns(p, frac, rng)
+    return S.apply_pinvar(p, p_inv)
+
+
+@pytest.mark.parametrize("n,T,sites", [(4, 4, 300), (4, 5, 200), (20, 4, 60)])
+def test_invar_brute_force(n, T, sites):
+    """logL with ptn_invar (lh = sum_c w_c L_c + invar, tree/phylokernelnew.h:3097) against
+    brute-force enumeration with the invariant term added."""
+    p = _invar_problem(n, T, sites, seed=n + T)
+    assert (p.invar > 0).sum() >= 5
+    o = S.Oracle(p)
+    a, b = 0, int(p.nei[0, 0])
+    lp_bf, tot_bf = S.brute_force_loglik(p, a, b)
+    tot, lp = o.loglik(a, b, with_patterns=True)
+    assert abs(tot - tot_bf) <= 1e-12 * abs(tot_bf)
+    np.testing.assert_allclose(lp, lp_bf, rtol=1e-12, atol=1e-12)
+    # the invariant term changes the answer (the check is not vacuous)
+    q = _invar_problem(n, T, sites, seed=n + T)
+    q.invar = None
+    assert abs(S.brute_force_loglik(q, a, b)[1] - tot_bf) > 1e-3
+
+
+@pytest.mark.parametrize("safe", [False, True])
+def test_invar_deep_tree_and_gate(safe):
+    """A deep caterpillar with constant columns: patterns with invariant mass are never
+    rescaled (the ptn_invar == 0 gate of tree/phylokernelnew.h:1999, 2020), their
+    partials may underflow and the invariant mass carries them; the oracle's logL equals
+    the renormalising pruning with the invariant term, and its scale counts equal the
+    rule-derived ones (zero on every pattern with invariant mass)."""
+    T = 200
+    p = _invar_problem(4, T, 150, seed=3, tree=S.caterpillar_tree(T, 1.5))
+    o = S.Oracle(p, safe=safe)
+    a, b = 0, int(p.nei[0, 0])
+    lp_ref, ref = S.pruning_loglik(p, a, b)
+    tot, lp = o.loglik(a, b, with_patterns=True)
+    assert abs(tot - ref) <= 1e-10 * abs(ref)
+    np.testing.assert_allclose(lp, lp_ref, rtol=1e-10, atol=1e-11)  # the all-unknown column has logL 0
+    bad, total, amb, top = _count_mismatches(o, p, safe, {})
+    assert bad == 0 and total > 0 and top >= 2
+    inv = p.invar > 0
+    for v in range(T, 2 * T - 2, 7):
+        d = int(p.nei[v, 0])
+        _, sc = o.partial(d, v, safe=safe)
+        sc = sc.reshape(p.nptn, -1)
+        assert not sc[inv].any()
+
+
+def test_ptn_invar_library_equals_definition():
+    """plk_ptn_invar (the computeConst/computePtnInvar coding) equals the definition
+    p_inv * sum_x pi_x prod_t [taxon t allows x] on DNA with ambiguity codes, protein with
+    B/Z/J and codons, and the hand-worked columns below."""
+    import gems_amd as G
+    for n, seed in ((4, 1), (20, 2), (61, 3)):
+        rng = np.random.default_rng(seed)
+        p = S.make_problem(n, 9, 400, rng, ambiguous=0.1)
+        S.plant_constant_columns(p, 0.4, rng)
+        want = S.ptn_invar_identity(p, 0.25)
+        got = G.ptn_invar(n, p.seq_type, p.states, p.pi, 0.25)
+        np.testing.assert_allclose(got, want, rtol=1e-15, atol=0)
+        assert (got > 0).sum() >= 0.3 * p.nptn and (got == 0).sum() > 0
+    pi = np.array([0.1, 0.2, 0.3, 0.4])
+    # columns: A,A | A,R(A|G = mask 5, code 8) | R,R | A,C | -,- | Y(C|T = mask 10, code 13),C
+    cols = np.array([[0, 0], [0, 8], [8, 8], [0, 1], [18, 18], [13, 1]], dtype=np.uint8).T
+    got = G.ptn_invar(4, S.SEQ_DNA, cols, pi, 0.5)
+    np.testing.assert_array_equal(got, 0.5 * np.array([0.1, 0.1, 0.1 + 0.3, 0.0, 1.0, 0.2]))
+    # protein: B (N|D) with N; Z with Z; J with K (nothing in common)
+    pi20 = np.full(20, 0.05)
+    cols = np.array([[20, 2], [21, 21], [22, 11]], dtype=np.uint8).T
+    got = G.ptn_invar(20, S.SEQ_PROTEIN, cols, pi20, 0.2)
+    np.testing.assert_allclose(got, 0.2 * np.array([0.05, 0.10, 0.0]), rtol=1e-15)
