@@ -19,6 +19,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the oracle's OpenMP team (CPU baseline leg) keeps its threads on neighbouring cores; read
+# by the OpenMP runtime when the oracle library is loaded
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 METRIC = "site·node partial-lh updates/s FP32 vs FP64 at stated logL err; p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
@@ -45,34 +48,164 @@ def profiled_traffic():
     return None
 
 
-def cpu_baseline(p, seconds=10.0, min_reps=2):
-    """Time the oracle's full traversal + branch logL on the host cores."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_cores():
+    """The cores this process may run on (the GPU box's CPU share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def time_oracle(p, threads, seconds, min_reps=2, max_reps=50, safe=False):
+    """Median wall time of the oracle's full traversal + branch logL on `threads` cores."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import support as S
 
-    threads = max(1, min(16, os.cpu_count() or 1))
-    o = S.Oracle(p, safe=False, nthreads=threads)
+    o = S.Oracle(p, safe=safe, nthreads=threads)
     a, b = 0, int(p.nei[0, 0])
     o.lib.orc_clear_all(o.h)
     ref = o.loglik(a, b)  # warm-up
     times = []
     t_start = time.perf_counter()
-    while len(times) < min_reps or (time.perf_counter() - t_start < seconds and len(times) < 50):
+    while len(times) < min_reps or (time.perf_counter() - t_start < seconds and len(times) < max_reps):
         o.lib.orc_clear_all(o.h)
         t0 = time.perf_counter()
         o.loglik(a, b)
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    upd = (p.ntaxa - 2) * p.nptn / med
     o.close()
+    return ref, float(np.median(times)), len(times)
+
+
+def cpu_baseline(p, seconds=10.0):
+    """The oracle (FP64 restatement of the reference kernel, pattern-interleaved SIMD lanes,
+    OpenMP packets) on every host core, threads bound close, and on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import support as S
+
+    cores = host_cores()
+    ref, med, n = time_oracle(p, cores, seconds)
+    _, med1, n1 = time_oracle(p, 1, seconds / 2, min_reps=1, max_reps=3)
+    upd = (p.ntaxa - 2) * p.nptn
     return ref, {
-        "value": upd,
+        "value": upd / med,
         "unit": "updates/s",
-        "cores": threads,
+        "cores": cores,
         "kind": "port",
-        "sample": f"{len(times)} full traversals + branch logL of the bench problem (median), "
-                  f"{S.oracle_lib().orc_lanes()}-lane SIMD oracle, OpenMP packets",
+        "sample": f"{n} full traversals + branch logL of the bench problem (median), "
+                  f"{S.oracle_lib().orc_lanes()}-lane SIMD oracle, OpenMP packets, OMP_PROC_BIND=close, "
+                  f"all {cores} cores of this host",
+        "value_1core": upd / med1,
+        "sample_1core": f"{n1} traversals on one core (median)",
+        "cpu_model": cpu_model(),
     }
+
+
+ALG_FLOPS = {4: 272, 20: 6480, 61: 59780}  # ncat(4n^2 + n) per update, ncat = 4
+PEAK = {"fp64_tflops": 78.6, "fp32_tflops": 157.3}
+
+
+def timed_traversals(G, p, precision, steps, scaling):
+    """Wall ms per full traversal + logL step and the traversal kernel's mean duration (HIP
+    events on the engine stream)."""
+    g = G.Tree(p, precision=precision, scaling=scaling)
+    g.full_traversal()
+    g.set_timing(True)
+    G.lib().plk_synchronize(g.context())
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        v = g.full_traversal()
+    G.lib().plk_synchronize(g.context())
+    el = (time.perf_counter() - t0) / steps
+    k_ms, k_n = g.timing(0)
+    g.close()
+    return el * 1e3, k_ms / max(k_n, 1), v
+
+
+def other_configs(G, no_cpu):
+    """Compact figures of the DNA (32 taxa x 10k sites), batched-partition (300 x 64 taxa)
+    and codon (48 taxa x 5k codons) workloads, both forms, each with its roofline fraction;
+    CPU oracle beside the DNA and codon ones on every host core."""
+    from gems_amd import workloads as W
+
+    res = {}
+    # DNA: HBM-bound unit (256 B FP64, 128 B FP32 per update)
+    p = W.config1()
+    upd = (p.ntaxa - 2) * p.nptn
+    r = {"workload": "dna4_G4_32taxa_10000sites_full_traversal", "patterns": p.nptn}
+    for prec, name, w in ((G.FP64, "fp64", 8), (G.FP32, "fp32", 4)):
+        ms, kms, v = timed_traversals(G, p, prec, 300, G.NORM)
+        bpu = 2 * 4 * 4 * w
+        r[name] = {"updates_per_s": upd / (ms * 1e-3), "ms_per_step": ms, "kernel_ms": kms, "logl": v,
+                   "roofline": {"bound": "hbm", "unit": "GB/s", "achieved": upd * bpu / (kms * 1e-3) / 1e9,
+                                "peak": HBM_PEAK_GBS, "frac": upd * bpu / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    r["fp32_over_fp64"] = r["fp32"]["updates_per_s"] / r["fp64"]["updates_per_s"]
+    if not no_cpu:
+        ref, med, n = time_oracle(p, host_cores(), 4.0)
+        r["cpu_updates_per_s"] = upd / med
+        r["cpu_cores"] = host_cores()
+        r["fp32_over_cpu"] = r["fp32"]["updates_per_s"] / r["cpu_updates_per_s"]
+        r["logl_rel_err"] = {k: abs(r[k]["logl"] - ref) / abs(ref) for k in ("fp64", "fp32")}
+    log(f"dna 32x10k: fp64 {r['fp64']['ms_per_step'] * 1e3:.1f} us, fp32 {r['fp32']['ms_per_step'] * 1e3:.1f} us")
+    res["dna_32taxa_10k"] = r
+
+    # batched partitions: bytes per update by state count, summed over the partitions
+    probs = W.config3()
+    upd = sum((q.ntaxa - 2) * q.nptn for q in probs)
+    br = [(0, int(q.nei[0, 0])) for q in probs]
+    r = {"workload": "300_unlinked_partitions_64taxa_70pct_dna", "patterns": int(sum(q.nptn for q in probs)),
+         "updates_per_traversal": upd}
+    for prec, name, w in ((G.FP64, "fp64", 8), (G.FP32, "fp32", 4)):
+        trees = [G.Tree(q, precision=prec) for q in probs]
+        b = G.Batch(trees)
+        b.autotune(br, reps=2)
+        b.loglik(br, clear=True)
+        kms0, kn0, _ = b.timing()  # cumulative event time of the batch's launches
+        steps = 20
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            vals, tot = b.loglik(br, clear=True)
+        el = (time.perf_counter() - t0) / steps
+        kms, kn, _ = b.timing()
+        kms = (kms - kms0) / max(kn - kn0, 1)
+        nbytes = sum((q.ntaxa - 2) * q.nptn * 2 * len(q.rates) * q.n * w for q in probs)
+        r[name] = {"updates_per_s": upd / el, "ms_per_step": el * 1e3, "kernel_ms": kms, "total_logl": tot,
+                   "roofline": {"bound": "hbm", "unit": "GB/s", "achieved": nbytes / (kms * 1e-3) / 1e9,
+                                "peak": HBM_PEAK_GBS, "frac": nbytes / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+        b.close()
+        for t in trees:
+            t.close()
+    log(f"batched 300 partitions: fp64 {r['fp64']['ms_per_step']:.2f} ms, fp32 {r['fp32']['ms_per_step']:.2f} ms")
+    res["batched_300_partitions"] = r
+
+    # codon: matrix-core (flop) bound, 59,780 algorithmic flops per update
+    p = W.config4()
+    upd = (p.ntaxa - 2) * p.nptn
+    r = {"workload": "codon61_G4_48taxa_5000codons_full_traversal", "patterns": p.nptn}
+    for prec, name, sc, pk in ((G.FP64, "fp64", G.SAFE, PEAK["fp64_tflops"]), (G.FP32, "fp32", G.NORM, PEAK["fp32_tflops"])):
+        ms, kms, v = timed_traversals(G, p, prec, 10, sc)
+        tf = upd * ALG_FLOPS[61] / (kms * 1e-3) / 1e12
+        r[name] = {"updates_per_s": upd / (ms * 1e-3), "ms_per_step": ms, "kernel_ms": kms, "logl": v,
+                   "roofline": {"bound": "mfma", "unit": "TFLOP/s", "achieved": tf, "peak": pk, "frac": tf / pk}}
+    r["fp32_over_fp64"] = r["fp32"]["updates_per_s"] / r["fp64"]["updates_per_s"]
+    if not no_cpu:
+        ref, med, n = time_oracle(p, host_cores(), 4.0, max_reps=3, safe=True)
+        r["cpu_updates_per_s"] = upd / med
+        r["cpu_cores"] = host_cores()
+        r["logl_rel_err"] = {k: abs(r[k]["logl"] - ref) / abs(ref) for k in ("fp64", "fp32")}
+    log(f"codon 48x5k: fp64 {r['fp64']['ms_per_step']:.2f} ms, fp32 {r['fp32']['ms_per_step']:.2f} ms")
+    res["codon_48taxa_5k"] = r
+    return res
 
 
 def main():
@@ -83,6 +216,7 @@ def main():
     ap.add_argument("--taxa", type=int, default=100)
     ap.add_argument("--sites", type=int, default=50000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the compact figures of the other workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -154,6 +288,28 @@ def main():
         log(f"cpu: {cpu['value'] / 1e9:.4f} G upd/s on {cpu['cores']} threads; |dlogL|/|logL| fp64 {err64:.2e}, "
             f"fp32 {err32:.2e}")
 
+    # df/ddf error of both forms against the oracle on two branches of this problem
+    deriv = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import support as S
+        o = S.Oracle(p, nthreads=host_cores())
+        deriv = {}
+        brs = [(0, int(p.nei[0, 0])), (p.ntaxa, int(p.nei[p.ntaxa, 0]))]
+        dref = [o.derivs(u, v)[:2] for (u, v) in brs]
+        o.close()
+        for prec, name in ((G.FP64, "fp64"), (G.FP32, "fp32")):
+            g = G.Tree(p, precision=prec)
+            e = 0.0
+            for (u, v), dd in zip(brs, dref):
+                dg = g.derivs(u, v)[:2]
+                e = max(e, max(abs(dg[i] - dd[i]) / max(1.0, abs(dd[i])) for i in range(2)))
+            g.close()
+            deriv[name] = e
+        log(f"derivatives: max rel err fp64 {deriv['fp64']:.2e}, fp32 {deriv['fp32']:.2e}")
+
+    others = None if args.no_configs else other_configs(G, args.no_cpu)
+
     out = {
         "metric": METRIC,
         "value": f32["ups"],
@@ -179,6 +335,17 @@ def main():
                      "roofline_frac": f64["achieved"] / HBM_PEAK_GBS},
             "fp32_over_fp64": f32["ups"] / f64["ups"],
             "gpu_fp32_over_cpu": (f32["ups"] / cpu["value"]) if cpu else None,
+            "deriv_max_rel_err_vs_oracle": deriv,
+            "other_workloads": others,
+        },
+        "value_fp64": f64["ups"],
+        "ms_per_step_fp64": f64["ms_per_step"],
+        "logl_rel_err_fp64": err64,
+        "logl_rel_err_fp32": err32,
+        "roofline_fp64": {
+            "bound": "hbm", "achieved": f64["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": f64["achieved"] / HBM_PEAK_GBS, "kernel": "k_traverse_f64<20,MODE_LOGL>",
+            "kernel_avg_ms": f64["kernel_avg_ms"], "alg_bytes_per_update": f64["bytes_per_update"],
         },
         "roofline": {
             "bound": "hbm", "achieved": f32["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 import gems_amd as G  # noqa: E402
+from gems_amd import workloads as W  # noqa: E402
 
 
 def log(*a):
@@ -135,7 +136,7 @@ def param_loop(p, precision, threads=None, gtr=False):
 
 
 def config1(quick):
-    p = G.Problem(G.SEQ_DNA, 32, 10000, mean_len=0.1, alpha=0.5, seed=1)
+    p = W.config1()
     log(f"config1: {p.nptn} patterns")
     steps = 50 if quick else 400
     r64 = traversal_rate(p, G.FP64, steps)
@@ -220,7 +221,7 @@ def rell_and_newton(p, o, thr, quick):
 
 
 def config2(quick):
-    p = G.Problem(G.SEQ_PROTEIN, 100, 50000, mean_len=0.2, alpha=0.8, seed=1)
+    p = W.config2()
     log(f"config2: {p.nptn} patterns")
     steps = 5 if quick else 20
     r64 = traversal_rate(p, G.FP64, steps)
@@ -248,15 +249,7 @@ def config2(quick):
 
 
 def config3(quick, nparts=300):
-    rng = np.random.default_rng(3)
-    base = G.Problem(G.SEQ_DNA, 64, 50, seed=3000)
-    probs = []
-    for i in range(nparts):
-        seq = G.SEQ_DNA if rng.random() < 0.7 else G.SEQ_PROTEIN
-        # lognormal, median 500 sites, sigma 1.0 (sigma is not fixed by the workload description)
-        nsites = int(np.clip(np.round(rng.lognormal(np.log(500), 1.0)), 100, 5000))
-        probs.append(G.Problem(seq, 64, nsites, mean_len=0.1 if seq == G.SEQ_DNA else 0.2,
-                               alpha=float(rng.uniform(0.3, 1.5)), seed=30000 + i, topology=base.nei))
+    probs = W.config3(nparts)
     upd = sum((p.ntaxa - 2) * p.nptn for p in probs)
     log(f"config3: {nparts} partitions, {sum(p.nptn for p in probs)} patterns, {upd} updates per traversal")
     br = [(0, int(p.nei[0, 0])) for p in probs]
@@ -290,7 +283,7 @@ def config3(quick, nparts=300):
 
 
 def config4(quick):
-    p = G.Problem(G.SEQ_CODON, 48, 5000, mean_len=0.1, alpha=0.5, seed=4)
+    p = W.config4()
     log(f"config4: {p.nptn} patterns")
     steps = 3 if quick else 10
     r64 = traversal_rate(p, G.FP64, steps, scaling=G.SAFE)
@@ -315,18 +308,10 @@ def config4(quick):
 
 
 def config5(quick, nreq=None):
-    rng = np.random.default_rng(5)
     nreq = nreq or (16 if quick else 40)
-    specs = []
-    for i in range(nreq):
-        seq = G.SEQ_DNA if rng.random() < 0.5 else G.SEQ_PROTEIN
-        T = int(np.exp(rng.uniform(np.log(16), np.log(512))))
-        sites = int(np.exp(rng.uniform(np.log(1000), np.log(100000))))
-        kind = G.REQ_LOGL if rng.random() < 0.5 else G.REQ_OPT_BRANCH
-        specs.append((seq, T, sites, kind, 500 + i))
+    specs, rng = W.config5_specs(nreq)
     t0 = time.perf_counter()
-    probs = [G.Problem(s, T, n, mean_len=0.1 if s == G.SEQ_DNA else 0.2, alpha=0.5 if s == G.SEQ_DNA else 0.8, seed=sd)
-             for (s, T, n, k, sd) in specs]
+    probs = [W.config5_problem(sp) for sp in specs]
     log(f"config5: generated {nreq} requests in {time.perf_counter() - t0:.1f}s")
     updates = sum((p.ntaxa - 2) * p.nptn for p in probs)
     out = {"requests": nreq, "updates_per_pass": updates, "sweep": {}}
@@ -334,7 +319,8 @@ def config5(quick, nreq=None):
     for prec, name in ((G.FP64, "fp64"), (G.FP32, "fp32")):
         res = []
         for rate in rates:
-            reqs = [G.Request(p, kind=k, precision=prec, opt_branch=(T, int(p.nei[T, 0])))
+            reqs = [G.Request(p, kind=G.REQ_LOGL if k == 0 else G.REQ_OPT_BRANCH, precision=prec,
+                              opt_branch=(T, int(p.nei[T, 0])))
                     for p, (s, T, n, k, sd) in zip(probs, specs)]
             gaps = rng.exponential(1.0 / rate, size=nreq)
             arrivals = np.cumsum(gaps) - gaps[0]
