@@ -90,6 +90,7 @@ _SIGS = {
     "plk_rell_eval": (_int, [_P, _P, _dp]),
     "plk_rell_save_tree": (_int, [_P, _P, _dbl, _dbl, _dp, _ip, ctypes.POINTER(_int)]),
     "plk_rell_get": (_int, [_P, _dp, _ip, _dp]),
+    "plk_rell_eval_noalloc": (_int, [_P, _int, _int, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     "plk_get_partial": (_int, [_P, _int, _dp, _u16p]),
     "plk_set_timing": (_int, [_P, _int]),
     "plk_get_timing": (_int, [_P, _int, _dp, _i64p]),

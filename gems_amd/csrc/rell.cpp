@@ -11,8 +11,8 @@
 
 namespace plk {
 
-hipError_t rell_launch(const double *ptn_lh, float *lhf, int nptn, int npad, const float *counts, int nsamples,
-                       double *rell, double *boot_logl, int *boot_counts, double *boot_orig, const double *uniform,
+hipError_t rell_launch(const double *ptn_lh, float *lhf, int nptn, int npad, const float *counts,
+                       int64_t count_stride, int nsamples, double *rell, double *boot_logl, int *boot_counts, double *boot_orig, const double *uniform,
                        int *improved, double cur_logl, double eps, bool update, hipStream_t s);
 
 #define RHIP_TRY(expr)                                                                          \
@@ -105,7 +105,7 @@ static int rell_run(plk_rell *r, plk_ctx *c, bool update, double cur_logl, doubl
         if (!uniform || !std::isfinite(eps) || eps < 0) return fail(PLK_ERR_ARG, "plk_rell_save_tree: bad arguments");
         RHIP_TRY(hipMemcpyAsync(r->d_uniform, uniform, sizeof(double) * r->nsamples, hipMemcpyHostToDevice, c->stream));
     }
-    RHIP_TRY(rell_launch(c->d_ptn_lh, r->d_lhf, r->nptn, r->npad, r->d_counts, r->nsamples, r->d_rell, r->d_logl,
+    RHIP_TRY(rell_launch(c->d_ptn_lh, r->d_lhf, r->nptn, r->npad, r->d_counts, r->npad, r->nsamples, r->d_rell, r->d_logl,
                          r->d_best_counts, r->d_orig, r->d_uniform, r->d_improved, cur_logl, eps, update, c->stream));
     return PLK_OK;
 }

@@ -49,6 +49,20 @@ int plk_rell_eval(plk_rell *r, plk_ctx *ctx, double *rell);
 int plk_rell_save_tree(plk_rell *r, plk_ctx *ctx, double cur_logl, double eps, const double *uniform,
                        int32_t *improved, int *nimproved);
 
+/* plk_rell_eval without any device allocation, for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, hosts that forbid hipMalloc).  No
+ * counterpart in the reference: it is plk_rell_eval on caller-owned device memory.
+ * d_counts: nsamples rows of pattern counts in device memory, row s at d_counts + s *
+ * count_stride (16-byte aligned, count_stride a multiple of 4 and >= nptn rounded up to 64;
+ * entries nptn .. that rounded size of each row are read, weighted by 0, and must be
+ * finite); d_rell: nsamples doubles of device memory.  The float copy of the per-pattern
+ * logL is staged in a library-owned static device array of 2^20 patterns (4 MiB per
+ * device, held until the process ends): nptn above 2^20 - 63 is rejected.  Every call on
+ * one device shares that array, so calls that overlap on one device race.  Runs on ctx's
+ * stream and returns when the scores are written. */
+int plk_rell_eval_noalloc(plk_ctx *ctx, int nsamples, int nptn, const float *d_counts, int64_t count_stride,
+                          double *d_rell);
+
 /* Read the replicate state (any pointer nullable). */
 int plk_rell_get(plk_rell *r, double *boot_logl, int32_t *boot_counts, double *boot_orig_logl);
 
