@@ -99,6 +99,7 @@ _SIGS = {
     "plk_get_exponents": (_int, [_P, _int, _ip]),
     "plk_set_tile_shape": (_int, [_P, _int]),
     "plk_set_level_waves": (_int, [_P, _int]),
+    "plk_set_f64_form": (_int, [_P, _int]),
     "plk_probe_copy": (_int, [ctypes.c_int64, _int, _dp]),
     # model / synthetic data
     "plk_state_unknown": (_int, [_int, _int]),
@@ -440,6 +441,9 @@ class Tree:
 
     def set_tile_shape(self, wpc):
         _check(lib().plk_set_tile_shape(self.context(), wpc))
+
+    def set_f64_form(self, form):
+        _check(lib().plk_set_f64_form(self.context(), form))
 
     def set_level_waves(self, waves):
         _check(lib().plk_set_level_waves(self.context(), waves))

@@ -32,11 +32,11 @@ namespace dev {
 constexpr int kRow61 = 62;                  // padded row length (doubles)
 constexpr int kLds61 = 4 * 61 * kRow61;     // doubles: four categories' matrices
 
-// echild of one side for every category, row-major [c][x][i] in memory, into E as [c][i][x]
+// echild of one side for every category, [c][i][x] in memory, into E as padded rows
 __device__ __forceinline__ void stage61_echild(const double *__restrict__ src, int ncat, double *E) {
     __syncthreads();  // every wavefront is done with the previous contents
     for (int idx = threadIdx.x; idx < ncat * 61 * 61; idx += blockDim.x) {
-        const int c = idx / (61 * 61), rem = idx - c * 61 * 61, x = rem / 61, i = rem - x * 61;
+        const int c = idx / (61 * 61), rem = idx - c * 61 * 61, i = rem / 61, x = rem - i * 61;
         E[(c * 61 + i) * kRow61 + x] = src[idx];
     }
     __syncthreads();
@@ -176,11 +176,11 @@ __device__ inline void pmatrix_f64_body(const KParams &P, const PJob &j, const i
         ex[c * n + i] = exp(P.eval[i] * (P.rates[c] * j.len));
     }
     __syncthreads();
-    // stored [c][i][x] (a column per i) for n <= 20, whose kernels read a column at a time
-    const bool cols = n <= 20;
+    // stored [c][i][x]: a column per i, x contiguous (matrix-core fragments and the
+    // column-at-a-time kernels read it that way)
     for (int idx = threadIdx.x; idx < nc * n * n; idx += blockDim.x) {
         const int c = cb + idx / (n * n), rem = idx % (n * n), x = rem / n, i = rem % n;
-        E[(int64_t)c * n * n + (cols ? i * n + x : rem)] = P.evec[x * n + i] * ex[c * n + i];
+        E[(int64_t)c * n * n + i * n + x] = P.evec[x * n + i] * ex[c * n + i];
     }
     if (j.leaf) {
         double *T = M + (int64_t)2 * ncat * n * n + (int64_t)j.side * (su + 1) * block;
@@ -240,8 +240,8 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
             double a = 0.0, b = 0.0;
 #pragma unroll
             for (int i = 0; i < N; i++) {
-                a = fma(eL[x * N + i], l[i], a);
-                b = fma(eR[x * N + i], r[i], b);
+                a = fma(eL[i * N + x], l[i], a);
+                b = fma(eR[i * N + x], r[i], b);
             }
             const double t = a * b;
 #pragma unroll
@@ -256,7 +256,7 @@ __device__ __forceinline__ void op_partial(const KParams &P, const DevOp &op, co
         for_states<N>([&](const int x) {
             double b = 0.0;
 #pragma unroll
-            for (int i = 0; i < N; i++) b = fma(eR[x * N + i], r[i], b);
+            for (int i = 0; i < N; i++) b = fma(eR[i * N + x], r[i], b);
             const double t = tl[x] * b;
 #pragma unroll
             for (int j = 0; j < N; j++) out[j] = fma(uinv[j * N + x], t, out[j]);

@@ -18,6 +18,8 @@ constexpr int f32_ksteps(int n) { return (n + 3) / 4; }
 // FP32 leaf-table row length: the matrix-core forms pad a row to whole 16-row blocks
 // (zeros past n), so a lane's four output rows are one aligned 16-byte load
 constexpr int f32_trow(int n) { return n == 4 ? 4 : f32_xblocks(n) * 16; }
+// FP64 matrix-core traversal: U^-1 as A fragments [16-row block][4-state k step][lane]
+constexpr int mfma64_uinv_frag_size(int n) { return ((n + 15) / 16) * ((n + 3) / 4) * 64; }
 // FP32 partial layout inside one (tile, category) block of 64 patterns.  20 states:
 // row groups of 4 states, element (x, p) at ((x / 4) * 64 + p) * 4 + x % 4, so the four
 // matrix-core output rows a lane holds (and the four B-operand rows it feeds) are one
@@ -109,6 +111,8 @@ struct KParams {
     const int32_t *tips;      // their taxon ids
     int br_lds_a, br_lds_b;   // LDS buffer (or staged tip index) of the branch's two sides
     int small_stage;          // level-parallel kernel: op list, level bounds and P(t) exponentials staged in LDS
+    const double *uinv_frag;  // FP64 matrix-core traversal: U^-1 as A fragments (null: not built)
+    int f64_form;             // FP64 20/61 states: 1 matrix-core traversal, 0 vector-ALU kernels
 };
 
 // Device-resident state of one Newton branch-length solve (Optimization::minimizeNewton,
@@ -163,6 +167,9 @@ struct Ctx {
     bool ptn_lh_valid = false;  // d_ptn_lh holds the per-pattern logL of the last logL evaluation
     bool timing = false;
     int wpc = 1;  // FP32 matrix-core tile shape (plk_set_tile_shape)
+    double *d_uinv_frag = nullptr;  // FP64 20/61 states: U^-1 as matrix-core fragments
+    int f64_form = -1;              // FP64 20/61 states, ncat <= 4: 1 matrix cores, 0 vector ALU,
+                                    // -1 by state count (61: matrix cores, 20: vector ALU)
     // 4-state models: ops per level run at once by the level-parallel kernel (0: the
     // category-split kernels; -1: chosen by size, see plk_set_level_waves)
     int small_waves = -1;

@@ -1,5 +1,6 @@
 // FP64 kernels for gfx950: thin __global__ wrappers over the bodies in dev_f64.h.
 #include "dev_f64.h"
+#include "dev_f64_mfma.h"
 
 namespace plk {
 namespace {
@@ -28,6 +29,13 @@ __global__ void __launch_bounds__(256, 1) k_traverse_f64_lds61(KParams P) {
     traverse_f64_body<61, MODE, false, true>(P, blockIdx.x);
 }
 
+// matrix-core traversal (20, 61 states, ncat <= 4): one workgroup per 16-pattern block,
+// one wavefront per 16-row block of the states
+template <int N>
+__global__ void __launch_bounds__(64 * MfmaShape<N>::RB) k_traverse_f64_mfma(KParams P) {
+    traverse_f64_mfma_body<N, MODE_PARTIALS>(P, blockIdx.x);
+}
+
 template <int N, int DERIV>
 __global__ void __launch_bounds__(256) k_theta_eval_f64(KParams P) {
     theta_eval_f64_body<N, DERIV>(P, blockIdx.x, P.br_len);
@@ -50,8 +58,23 @@ __global__ void __launch_bounds__(256) k_theta_newton_f64(KParams P, NewtonState
 }
 
 template <int N>
-hipError_t traverse_n(const KParams &p, int mode, hipStream_t s) {
+hipError_t traverse_n(const KParams &p0, int mode, hipStream_t s) {
+    KParams p = p0;
     dim3 grid(p.ntiles), block(64 * p.ncw);
+    if constexpr (N != 4) {
+        if (p.f64_form == 1 && p.ncat <= 4 && p.uinv_frag) {
+            // the op list on the matrix cores, then the branch stage alone (no ops) in the
+            // vector kernel
+            if (p.nops > 0) {
+                hipLaunchKernelGGL((k_traverse_f64_mfma<N>), dim3(p.ntiles * 4), dim3(64 * MfmaShape<N>::RB), 0, s, p);
+                if (mode == MODE_PARTIALS) return hipGetLastError();
+                p.nops = 0;
+            }
+            if (mode == MODE_LOGL) hipLaunchKernelGGL((k_traverse_f64<N, MODE_LOGL>), grid, block, 0, s, p);
+            else if (mode == MODE_THETA) hipLaunchKernelGGL((k_traverse_f64<N, MODE_THETA>), grid, block, 0, s, p);
+            return hipGetLastError();
+        }
+    }
     if (N == 61 && p.ncat <= 4) {
         if (mode == MODE_PARTIALS) hipLaunchKernelGGL((k_traverse_f64_lds61<MODE_PARTIALS>), grid, block, 0, s, p);
         else if (mode == MODE_LOGL) hipLaunchKernelGGL((k_traverse_f64_lds61<MODE_LOGL>), grid, block, 0, s, p);

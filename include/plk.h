@@ -165,6 +165,12 @@ int plk_get_exponents(plk_ctx *ctx, int slot, int32_t *exponents);
  * parallelism; fewer reuse each matrix fragment over more patterns).  Results do not
  * depend on it. */
 int plk_set_tile_shape(plk_ctx *ctx, int waves_per_category);
+/* FP64 contexts with 20 or 61 states and at most four categories: form 1 runs the
+ * partial-likelihood ops on the matrix cores (FP64 16x16x4 tiles, one workgroup per 16
+ * patterns); form 0 the vector-ALU kernels; -1 (default) picks by state count, the faster
+ * one measured (61 states: matrix cores, 20: vector ALU).  Same results to rounding (the
+ * sums run in a different order); scale counts follow the same rule. */
+int plk_set_f64_form(plk_ctx *ctx, int form);
 
 /* 4-state models: 1..4 = level-parallel kernel: the ops are sorted into dependency
  * levels and a 64-pattern tile runs that many independent ops of a level at once
