@@ -40,8 +40,11 @@ struct MfmaShape {
     static constexpr int KS = (N + 3) / 4;    // 4-state k steps
 };
 
-template <int N, int MODE>
-__device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk) {
+// Categories c0 .. c1-1 of the block: all of them under NORM scaling (one check over the
+// pattern's categories), or one per workgroup under SAFE (each category checked alone).
+// NC: categories a workgroup holds at most (1 for the per-category SAFE launch, 4 otherwise).
+template <int N, int MODE, int NC>
+__device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk, const int c0, const int c1) {
     constexpr int RB = MfmaShape<N>::RB, KS = MfmaShape<N>::KS, ROWS = RB * 16;
     const int lane = threadIdx.x & 63;
     const int rb = wave_id();
@@ -54,11 +57,11 @@ __device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk) {
     uint16_t *__restrict__ scale = (uint16_t *)P.scale;
     const double invar = P.invar[p];
     // U^-1 as A fragments [rb][ks][lane] = U^-1[16 rb + lane % 16][4 ks + lane / 16], zero
-    // outside n x n (built on the host by plk_set_model)
+    // outside n x n (built on the host by plk_set_model); read per use (L1/L2 resident)
     const double *__restrict__ uf = P.uinv_frag + (int64_t)rb * KS * 64 + lane;
 
-    __shared__ double s_t[4][ROWS][16];  // a o b of each category, [c][x][pattern]
-    __shared__ double s_m[RB][4][16];    // per row wave: max |out| per category and pattern
+    __shared__ double s_t[NC][ROWS][16];  // a o b of each category, [c - c0][x][pattern]
+    __shared__ double s_m[RB][NC][16];    // per row wave: max |out| per category and pattern
 
     for (int k = 0; k < P.nops; k++) {
         const int ko = op_index(P, tile, k);
@@ -70,7 +73,7 @@ __device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk) {
             int sl = 0, sr = 0;
             if (KIND <= KIND_TI) sl = P.states[(int64_t)op.left * P.nptn_pad + p];
             if (KIND == KIND_TT) sr = P.states[(int64_t)op.right * P.nptn_pad + p];
-            for (int c = 0; c < ncat; c++) {
+            for (int c = c0; c < c1; c++) {
                 const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + pl;
                 // one side: E * child over the k steps, or the leaf-table rows of a tip
                 auto side = [&](const double *__restrict__ E, int slot, const double *__restrict__ tab) {
@@ -85,15 +88,21 @@ __device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk) {
                     }
                     const double *__restrict__ C = part + (int64_t)slot * P.slot_stride + off;
                     const int x = rb * 16 + col;
-                    double av[KS], bv[KS];
+                    // k steps in chunks of up to 8 (operands of a chunk loaded before its tiles)
+                    constexpr int CH = KS < 8 ? KS : 8;
 #pragma unroll
-                    for (int ks = 0; ks < KS; ks++) {
-                        const int i = 4 * ks + g;
-                        av[ks] = (x < N && i < N) ? E[i * N + x] : 0.0;
-                        bv[ks] = i < N ? C[i * 64] : 0.0;
+                    for (int k0 = 0; k0 < KS; k0 += CH) {
+                        double av[CH], bv[CH];
+#pragma unroll
+                        for (int q = 0; q < CH; q++) {
+                            const int i = 4 * (k0 + q) + g;
+                            av[q] = (k0 + q < KS && x < N && i < N) ? E[i * N + x] : 0.0;
+                            bv[q] = (k0 + q < KS && i < N) ? C[i * 64] : 0.0;
+                        }
+#pragma unroll
+                        for (int q = 0; q < CH; q++)
+                            if (k0 + q < KS) d = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], bv[q], d, 0, 0, 0);
                     }
-#pragma unroll
-                    for (int ks = 0; ks < KS; ks++) d = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], d, 0, 0, 0);
                     return d;
                 };
                 const double *EL = M + (int64_t)c * N * N;
@@ -102,58 +111,57 @@ __device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk) {
                 const double4_t b = side(ER, op.right,
                                          KIND == KIND_TT ? tabs + (int64_t)(P.su + 1 + sr) * ncat * N + c * N : nullptr);
 #pragma unroll
-                for (int r = 0; r < 4; r++) s_t[c][rb * 16 + 4 * r + g][col] = a[r] * b[r];
+                for (int r = 0; r < 4; r++) s_t[c - c0][rb * 16 + 4 * r + g][col] = a[r] * b[r];
             }
             __syncthreads();
 
             // out = U^-1 (a o b): this wave's 16 output rows j = 16 rb + 4 g + r
-            double4_t out[4];
-            double uv[KS];
+            double4_t out[NC];
+            const double *ufo = uf;  // re-read per op, not held in registers across ops
+            asm volatile("" : "+v"(ufo));
 #pragma unroll
-            for (int ks = 0; ks < KS; ks++) uv[ks] = uf[ks * 64];
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                out[c] = double4_t{0.0, 0.0, 0.0, 0.0};
-                if (c >= ncat) continue;
+            for (int cc = 0; cc < NC; cc++) {
+                out[cc] = double4_t{0.0, 0.0, 0.0, 0.0};
+                if (c0 + cc >= c1) continue;
 #pragma unroll
                 for (int ks = 0; ks < KS; ks++)
-                    out[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(uv[ks], s_t[c][4 * ks + g][col], out[c], 0, 0, 0);
+                    out[cc] = __builtin_amdgcn_mfma_f64_16x16x4f64(ufo[ks * 64], s_t[cc][4 * ks + g][col], out[cc], 0, 0, 0);
                 double m = 0.0;
 #pragma unroll
-                for (int r = 0; r < 4; r++) m = fmax(m, fabs(out[c][r]));
+                for (int r = 0; r < 4; r++) m = fmax(m, fabs(out[cc][r]));
                 m = fmax(m, __shfl_xor(m, 16, 64));
                 m = fmax(m, __shfl_xor(m, 32, 64));
-                if (g == 0) s_m[rb][c][col] = m;
+                if (g == 0) s_m[rb][cc][col] = m;
             }
             __syncthreads();
 
             // scaling decision per pattern (NORM) or per pattern and category (SAFE)
-            bool up[4];
+            bool up[NC];
             if (!P.safe) {
                 double pm = 0.0;
                 for (int w = 0; w < RB; w++)
-                    for (int c = 0; c < ncat; c++) pm = fmax(pm, s_m[w][c][col]);
+                    for (int cc = 0; cc < c1 - c0; cc++) pm = fmax(pm, s_m[w][cc][col]);
                 const bool u = KIND != KIND_TT && pm < kST && invar == 0.0;
 #pragma unroll
-                for (int c = 0; c < 4; c++) up[c] = u;
+                for (int cc = 0; cc < NC; cc++) up[cc] = u;
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                for (int cc = 0; cc < NC; cc++) {
                     double pm = 0.0;
-                    if (c < ncat)
-                        for (int w = 0; w < RB; w++) pm = fmax(pm, s_m[w][c][col]);
-                    up[c] = KIND != KIND_TT && pm < kST && invar == 0.0;
+                    if (c0 + cc < c1)
+                        for (int w = 0; w < RB; w++) pm = fmax(pm, s_m[w][cc][col]);
+                    up[cc] = KIND != KIND_TT && pm < kST && invar == 0.0;
                 }
             }
             double *__restrict__ outp = part + (int64_t)op.out_slot * P.slot_stride;
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                if (c >= ncat) continue;
-                const int64_t off = ((int64_t)tile * ncat + c) * N * 64 + pl;
+            for (int cc = 0; cc < NC; cc++) {
+                if (c0 + cc >= c1) continue;
+                const int64_t off = ((int64_t)tile * ncat + c0 + cc) * N * 64 + pl;
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int j = rb * 16 + 4 * r + g;
-                    if (j < N) outp[off + j * 64] = up[c] ? out[c][r] * kTwo256 : out[c][r];
+                    if (j < N) outp[off + j * 64] = up[cc] ? out[cc][r] * kTwo256 : out[cc][r];
                 }
             }
             // scale counts: s_out = children's counts (+1 where rescaled), 1643-1644,
@@ -166,13 +174,15 @@ __device__ inline void traverse_f64_mfma_body(const KParams &P, const int blk) {
                         sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + p] + scale[(int64_t)op.right * P.scale_stride + p];
                     scale[(int64_t)op.out_slot * P.scale_stride + p] = (uint16_t)(sc + (up[0] ? 1u : 0u));
                 } else {
-                    for (int c = 0; c < ncat; c++) {
-                        const int64_t a = p * ncat + c;
+#pragma unroll
+                    for (int cc = 0; cc < NC; cc++) {
+                        if (c0 + cc >= c1) continue;
+                        const int64_t a = p * ncat + c0 + cc;
                         unsigned sc = 0;
                         if (KIND == KIND_TI) sc = scale[(int64_t)op.right * P.scale_stride + a];
                         else if (KIND == KIND_II)
                             sc = (unsigned)scale[(int64_t)op.left * P.scale_stride + a] + scale[(int64_t)op.right * P.scale_stride + a];
-                        scale[(int64_t)op.out_slot * P.scale_stride + a] = (uint16_t)(sc + (up[c] ? 1u : 0u));
+                        scale[(int64_t)op.out_slot * P.scale_stride + a] = (uint16_t)(sc + (up[cc] ? 1u : 0u));
                     }
                 }
             }

@@ -31,9 +31,20 @@ __global__ void __launch_bounds__(256, 1) k_traverse_f64_lds61(KParams P) {
 
 // matrix-core traversal (20, 61 states, ncat <= 4): one workgroup per 16-pattern block,
 // one wavefront per 16-row block of the states
-template <int N>
-__global__ void __launch_bounds__(64 * MfmaShape<N>::RB) k_traverse_f64_mfma(KParams P) {
-    traverse_f64_mfma_body<N, MODE_PARTIALS>(P, blockIdx.x);
+// CSPLIT (SAFE scaling only): one workgroup per (block, category), four times the
+// wavefronts for the same work
+// Register budget measured on the 48-taxon x 3,962-pattern codon traversal: the
+// per-category launch runs best at four wavefronts per SIMD (128 VGPRs, a few spilled;
+// 0.58 ms against 0.60 at three and 0.62 at two), the all-category (NORM) one at two
+// (0.76 ms against 0.79 at three and 1.00 at four).
+template <int N, bool CSPLIT>
+__global__ void __launch_bounds__(64 * MfmaShape<N>::RB, CSPLIT ? 4 : 2) k_traverse_f64_mfma(KParams P) {
+    if constexpr (CSPLIT) {
+        const int c = blockIdx.x % P.ncat;
+        traverse_f64_mfma_body<N, MODE_PARTIALS, 1>(P, blockIdx.x / P.ncat, c, c + 1);
+    } else {
+        traverse_f64_mfma_body<N, MODE_PARTIALS, 4>(P, blockIdx.x, 0, P.ncat);
+    }
 }
 
 template <int N, int DERIV>
@@ -66,7 +77,12 @@ hipError_t traverse_n(const KParams &p0, int mode, hipStream_t s) {
             // the op list on the matrix cores, then the branch stage alone (no ops) in the
             // vector kernel
             if (p.nops > 0) {
-                hipLaunchKernelGGL((k_traverse_f64_mfma<N>), dim3(p.ntiles * 4), dim3(64 * MfmaShape<N>::RB), 0, s, p);
+                if (p.safe)
+                    hipLaunchKernelGGL((k_traverse_f64_mfma<N, true>), dim3(p.ntiles * 4 * p.ncat),
+                                       dim3(64 * MfmaShape<N>::RB), 0, s, p);
+                else
+                    hipLaunchKernelGGL((k_traverse_f64_mfma<N, false>), dim3(p.ntiles * 4), dim3(64 * MfmaShape<N>::RB),
+                                       0, s, p);
                 if (mode == MODE_PARTIALS) return hipGetLastError();
                 p.nops = 0;
             }
