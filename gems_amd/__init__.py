@@ -164,6 +164,10 @@ _SIGS = {
     "plk_server_now": (ctypes.c_double, [_P]),
     "plk_server_stats": (_int, [_P, _dp]),
     "plk_server_destroy": (_int, [_P]),
+    "plk_server_counters": (_int, [_P, _i64p]),
+    "plk_reshape": (_int, [_P, _int, _int, _int]),
+    "plk_tree_reshape": (_int, [_P, _int, _int]),
+    "plk_batch_rebind": (_int, [_P, ctypes.POINTER(_P), _int]),
 }
 
 
@@ -576,6 +580,12 @@ class Server:
         _check(lib().plk_server_stats(self.h, _d(st)))
         keys = ["completed", "failed", "p50_s", "p99_s", "mean_s", "makespan_s", "peak_hbm_bytes", "peak_concurrent"]
         return dict(zip(keys, st.tolist()))
+
+    def counters(self):
+        """(shared traversal launches, request traversals run in them, pooled trees)"""
+        out = np.zeros(3, dtype=np.int64)
+        _check(lib().plk_server_counters(self.h, out.ctypes.data_as(_i64p)))
+        return {"launches": int(out[0]), "batched": int(out[1]), "pooled_trees": int(out[2])}
 
     def close(self):
         if getattr(self, "h", None):

@@ -25,6 +25,7 @@ namespace plk {
 
 struct Batch {
     std::vector<plk_ctx *> parts;
+    int parts_cap = 0, items_cap = 0;  // allocated d_params/d_results/h_results and d_items
     int prec = PLK_FP64;
     hipStream_t stream = nullptr;
     KParams *d_params = nullptr;
@@ -72,19 +73,78 @@ struct plk_batch : public plk::Batch {};
 
 using namespace plk;
 
+static int check_parts(plk_ctx *const *parts, int nparts, const char *who) {
+    const int prec = parts[0] ? parts[0]->prec : 0;
+    for (int i = 0; i < nparts; i++) {
+        plk_ctx *c = parts[i];
+        if (!c) return fail(PLK_ERR_ARG, std::string(who) + ": null context");
+        if (c->prec != prec) return fail(PLK_ERR_ARG, std::string(who) + ": partitions must share one precision form");
+        if (c->n != 4 && c->n != 20) return fail(PLK_ERR_ARG, std::string(who) + ": batched partitions are DNA or 20-state");
+        if (c->ncat > 4) return fail(PLK_ERR_ARG, std::string(who) + ": batched partitions need ncat <= 4");
+        for (int j = 0; j < i; j++)
+            if (parts[j] == c) return fail(PLK_ERR_ARG, std::string(who) + ": a context appears twice");
+    }
+    return PLK_OK;
+}
+
+// all tiles of all partitions, most expensive partitions first
+static std::vector<BatchItem> make_items(plk_ctx *const *parts, int nparts) {
+    std::vector<int> order(nparts);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int i) { return (double)parts[i]->n * parts[i]->n * parts[i]->ntiles; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::vector<BatchItem> items;
+    for (int i : order)
+        for (int t = 0; t < parts[i]->ntiles; t++) items.push_back(BatchItem{i, t});
+    return items;
+}
+
+extern "C" int plk_batch_rebind(plk_batch *b, plk_ctx *const *parts, int nparts) {
+    clear_error();
+    if (!b || !parts || nparts < 1) return fail(PLK_ERR_ARG, "plk_batch_rebind: bad arguments");
+    int rc = check_parts(parts, nparts, "plk_batch_rebind");
+    if (rc) return rc;
+    if (parts[0]->prec != b->prec) return fail(PLK_ERR_ARG, "plk_batch_rebind: the batch's precision form differs");
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    drop_graph(*b);
+    const std::vector<BatchItem> items = make_items(parts, nparts);
+    if (nparts > b->parts_cap) {
+        const int cap = std::max(nparts, 2 * b->parts_cap);
+        // (plain hipFree/hipMalloc like plk_batch_create; growth doubles, so this is rare)
+        if (b->d_params) (void)hipFree(b->d_params);
+        if (b->d_results) (void)hipFree(b->d_results);
+        if (b->h_results) (void)hipHostFree(b->h_results);
+        b->d_params = nullptr;
+        b->d_results = b->h_results = nullptr;
+        b->parts_cap = 0;
+        BHIP_TRY(hipMalloc((void **)&b->d_params, sizeof(KParams) * cap));
+        BHIP_TRY(hipMalloc((void **)&b->d_results, sizeof(double) * 2 * cap));
+        BHIP_TRY(hipHostMalloc((void **)&b->h_results, sizeof(double) * 2 * cap, hipHostMallocDefault));
+        b->parts_cap = cap;
+    }
+    if ((int)items.size() > b->items_cap) {
+        const int cap = std::max((int)items.size(), 2 * b->items_cap);
+        if (b->d_items) (void)hipFree(b->d_items);
+        b->d_items = nullptr;
+        b->items_cap = 0;
+        BHIP_TRY(hipMalloc((void **)&b->d_items, sizeof(BatchItem) * cap));
+        b->items_cap = cap;
+    }
+    BHIP_TRY(hipMemcpyAsync(b->d_items, items.data(), sizeof(BatchItem) * items.size(), hipMemcpyHostToDevice, b->stream));
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    b->parts.assign(parts, parts + nparts);
+    b->nitems = (int)items.size();
+    return PLK_OK;
+}
+
 extern "C" int plk_batch_create(plk_batch **out, plk_ctx *const *parts, int nparts) {
     clear_error();
     if (!out || !parts || nparts < 1) return fail(PLK_ERR_ARG, "plk_batch_create: bad arguments");
     *out = nullptr;
     const int prec = parts[0] ? parts[0]->prec : 0;
-    for (int i = 0; i < nparts; i++) {
-        plk_ctx *c = parts[i];
-        if (!c) return fail(PLK_ERR_ARG, "plk_batch_create: null context");
-        if (c->prec != prec) return fail(PLK_ERR_ARG, "plk_batch_create: partitions must share one precision form");
-        if (c->n != 4 && c->n != 20) return fail(PLK_ERR_ARG, "plk_batch_create: batched partitions are DNA or 20-state");
-        if (c->ncat > 4) return fail(PLK_ERR_ARG, "plk_batch_create: batched partitions need ncat <= 4");
-        for (int j = 0; j < i; j++)
-            if (parts[j] == c) return fail(PLK_ERR_ARG, "plk_batch_create: a context appears twice");
+    {
+        int rc = check_parts(parts, nparts, "plk_batch_create");
+        if (rc) return rc;
     }
     auto *b = new plk_batch();
     b->parts.assign(parts, parts + nparts);
@@ -93,15 +153,10 @@ extern "C" int plk_batch_create(plk_batch **out, plk_ctx *const *parts, int npar
         plk_batch_destroy(b);
         return rc;
     };
-    // items: all tiles of all partitions, most expensive partitions first
-    std::vector<int> order(nparts);
-    std::iota(order.begin(), order.end(), 0);
-    auto cost = [&](int i) { return (double)parts[i]->n * parts[i]->n * parts[i]->ntiles; };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
-    std::vector<BatchItem> items;
-    for (int i : order)
-        for (int t = 0; t < parts[i]->ntiles; t++) items.push_back(BatchItem{i, t});
+    const std::vector<BatchItem> items = make_items(parts, nparts);
     b->nitems = (int)items.size();
+    b->parts_cap = nparts;
+    b->items_cap = b->nitems;
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&b->d_params, sizeof(KParams) * nparts)) != hipSuccess ||

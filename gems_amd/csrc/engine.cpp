@@ -551,6 +551,12 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMallocAsync((void **)&c->d_counter, sizeof(unsigned), c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_sink, kSinkBytes, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_newton, sizeof(NewtonState), c->stream));
+    c->cap_part = rs * slot * nslots;
+    c->cap_scale = 2 * scale_slot * nslots;
+    c->cap_states = (size_t)ntaxa * c->nptn_pad;
+    c->cap_pad = sizeof(double) * c->nptn_pad;
+    c->cap_theta = sizeof(double) * slot;
+    c->cap_tiles = sizeof(double) * 2 * 4 * c->ntiles;
     const int n = nstates;
     const size_t model_doubles = n + 2 * n * n + 2 * ncat + 2 * (size_t)(c->su + 1) * n + n + n * n;
     CREATE_TRY(hipMallocAsync((void **)&c->d_model, sizeof(double) * model_doubles, c->stream));
@@ -600,6 +606,75 @@ extern "C" int plk_destroy(plk_ctx *c) {
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    return PLK_OK;
+}
+
+// Grow one stream-ordered buffer to at least `need` bytes (contents are not kept).
+static int grow(Ctx &c, void **buf, size_t &cap, size_t need) {
+    if (need <= cap) return PLK_OK;
+    if (*buf) (void)hipFreeAsync(*buf, c.stream);
+    *buf = nullptr;
+    cap = 0;
+    HIP_TRY(hipMallocAsync(buf, need, c.stream));
+    cap = need;
+    return PLK_OK;
+}
+
+extern "C" int plk_reshape(plk_ctx *c, int ntaxa, int nptn, int nslots) {
+    clear_error();
+    if (!c) return fail(PLK_ERR_ARG, "plk_reshape: null context");
+    if (ntaxa < 3 || nptn < 1 || nslots < 1) return fail(PLK_ERR_ARG, "plk_reshape: need ntaxa >= 3, nptn >= 1, nslots >= 1");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int pad = (int)round_up(nptn, 64), ntiles = pad / 64;
+    const size_t rs = c->real_size();
+    const int64_t slot = (int64_t)pad * c->block;
+    const int64_t scale_slot = c->prec == PLK_FP32 ? (int64_t)pad * 2 : (int64_t)pad * (c->safe ? c->ncat : 1);
+    int rc = grow(*c, (void **)&c->d_part, c->cap_part, rs * slot * nslots);
+    if (!rc) rc = grow(*c, (void **)&c->d_scale, c->cap_scale, 2 * scale_slot * nslots);
+    if (!rc) rc = grow(*c, (void **)&c->d_states, c->cap_states, (size_t)ntaxa * pad);
+    if (!rc && sizeof(double) * pad > c->cap_pad) {
+        size_t cf = c->cap_pad, ci = c->cap_pad, cb = c->cap_pad, cl = c->cap_pad;
+        rc = grow(*c, (void **)&c->d_freq, cf, sizeof(double) * pad);
+        if (!rc) rc = grow(*c, (void **)&c->d_invar, ci, sizeof(double) * pad);
+        if (!rc) rc = grow(*c, (void **)&c->d_buf_scale, cb, sizeof(double) * pad);
+        if (!rc) rc = grow(*c, (void **)&c->d_ptn_lh, cl, sizeof(double) * pad);
+        if (!rc) c->cap_pad = sizeof(double) * pad;
+    }
+    if (!rc) rc = grow(*c, (void **)&c->d_theta, c->cap_theta, sizeof(double) * slot);
+    if (!rc) rc = grow(*c, (void **)&c->d_tile_sums, c->cap_tiles, sizeof(double) * 2 * 4 * ntiles);
+    if (rc) return rc;
+    if (ntiles != c->ntiles && c->mcap) {
+        // the many-branch buffers hold (branch, tile) items: rebuilt on next use
+        void *old[] = {c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres, c->d_mcount, c->d_mnewton};
+        for (void *q : old)
+            if (q) (void)hipFreeAsync(q, c->stream);
+        if (c->h_mnewton) (void)hipHostFree(c->h_mnewton);
+        c->d_mparams = c->d_mitems = nullptr;
+        c->d_mtile = c->d_mres = nullptr;
+        c->d_mcount = nullptr;
+        c->d_mnewton = c->h_mnewton = nullptr;
+        c->mcap = 0;
+    }
+    // the op buffer also holds level bounds and up to ntaxa + 1 tip ids: rebuilt when the
+    // taxon count grows
+    if (ntaxa > c->ntaxa) c->ops_cap = c->mats_cap = 0;
+    c->ntaxa = ntaxa;
+    c->nptn = nptn;
+    c->nptn_pad = pad;
+    c->ntiles = ntiles;
+    c->nslots = nslots;
+    HIP_TRY(hipMemsetAsync(c->d_scale, 0, 2 * scale_slot * nslots, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_freq, 0, sizeof(double) * pad, c->stream));
+    HIP_TRY(hipMemsetAsync(c->d_invar, 0, sizeof(double) * pad, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    rc = ensure_op_capacity(*c, std::max(16, ntaxa + 1));
+    if (rc) return rc;
+    c->pending.clear();
+    c->jobs.clear();
+    c->host_freq.clear();
+    c->patterns_set = false;
+    c->theta_valid = false;
+    c->ptn_lh_valid = false;
     return PLK_OK;
 }
 

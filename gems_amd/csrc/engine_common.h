@@ -179,6 +179,8 @@ struct Ctx {
     unsigned *d_mcount = nullptr;
     NewtonState *d_mnewton = nullptr, *h_mnewton = nullptr;  // many-branch Newton states (pinned mirror)
     int mcap = 0;
+    // allocated sizes (bytes) of the pattern-sized buffers, for plk_reshape
+    size_t cap_part = 0, cap_scale = 0, cap_states = 0, cap_pad = 0, cap_theta = 0, cap_tiles = 0;
     hipEvent_t ev[6] = {};
     double t_total[3] = {0, 0, 0};
     int64_t t_count[3] = {0, 0, 0};

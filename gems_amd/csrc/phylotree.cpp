@@ -728,6 +728,27 @@ extern "C" int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa,
     return plk_tree_create_ex(out, nstates, ncat, ntaxa, nptn, precision, scaling, 0);
 }
 
+extern "C" int plk_tree_reshape(plk_tree *t, int ntaxa, int nptn) {
+    if (!t) return fail(PLK_ERR_ARG, "plk_tree_reshape: null tree");
+    if (t->all_dirs) return fail(PLK_ERR_STATE, "plk_tree_reshape: trees with one slot per direction are not reshaped");
+    if (ntaxa < 3) return fail(PLK_ERR_ARG, "plk_tree_reshape: need at least 3 taxa");
+    int rc = plk_reshape(t->ctx, ntaxa, nptn, ntaxa - 2);
+    if (rc) return rc;
+    t->ntaxa = ntaxa;
+    t->nnodes = 2 * ntaxa - 2;
+    t->nni_base = 3 * (ntaxa - 2);
+    t->nei.assign((size_t)t->nnodes * 3, -1);
+    t->ssize.assign((size_t)t->nnodes * 3, -1);
+    t->len.assign((size_t)t->nnodes * 3, 0.0);
+    t->orient.assign(t->nnodes, -1);
+    t->computed.assign((size_t)t->nnodes, 0);
+    t->ops.clear();
+    t->model_stored = false;
+    t->gamma_shape = 0.0;
+    t->gtr_set = false;
+    return PLK_OK;
+}
+
 extern "C" int plk_tree_destroy(plk_tree *t) {
     if (!t) return PLK_OK;
     plk_destroy(t->ctx);

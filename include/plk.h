@@ -82,6 +82,12 @@ int plk_device_units(void);
 int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int nptn, int nslots,
                int precision, int scaling);
 int plk_destroy(plk_ctx *ctx);
+/* Reuse a context for ntaxa taxa, nptn patterns and nslots slots (same state count,
+ * categories, precision, scaling): device buffers are kept where large enough and grown
+ * (stream-ordered) otherwise; patterns must be set again, the model stays.  The
+ * re-initialisation of a PhyloTree's buffers for a new alignment (initializeAllPartialLh,
+ * tree/phylotree.cpp:905-960) without giving the memory back. */
+int plk_reshape(plk_ctx *ctx, int ntaxa, int nptn, int nslots);
 /* Bytes of device memory a context of this shape holds (PhyloTree::getMemoryRequired,
  * tree/phylotree.cpp:992-1052). */
 int64_t plk_memory_required(int nstates, int ncat, int ntaxa, int nptn, int nslots,

@@ -25,6 +25,11 @@ typedef struct plk_batch plk_batch;
 /* All contexts share one precision form; state counts 4 or 20, ncat <= 4. */
 int plk_batch_create(plk_batch **out, plk_ctx *const *parts, int nparts);
 int plk_batch_destroy(plk_batch *b);
+/* Replace the batch's partitions (same rules as plk_batch_create), keeping its device and
+ * staging buffers where they are large enough: the serving loop re-forms its batch from
+ * the requests in flight at every step (PhyloSuperTree's partition loop over a changing
+ * set, tree/phylosupertree.cpp:709-734).  Drops any captured graph. */
+int plk_batch_rebind(plk_batch *b, plk_ctx *const *parts, int nparts);
 /* grid: persistent workgroups (0 = one per work item); use_graph: replay a captured graph. */
 int plk_batch_set_launch(plk_batch *b, int grid, int use_graph);
 /* Per partition i: run its queued ops and the branch logL for node refs a[i], b[i] at

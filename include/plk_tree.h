@@ -34,6 +34,11 @@ int plk_tree_create(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, 
 int plk_tree_create_ex(plk_tree **out, int nstates, int ncat, int ntaxa, int nptn, int precision, int scaling,
                        int all_directions);
 int plk_tree_destroy(plk_tree *t);
+/* Reuse a tree (and its context's device buffers) for another problem of the same state
+ * count, categories, precision and scaling: ntaxa taxa and nptn patterns.  Buffers are
+ * kept where they are large enough and grown otherwise; model, patterns and topology
+ * must be set again.  Not for trees created with all_directions. */
+int plk_tree_reshape(plk_tree *t, int ntaxa, int nptn);
 plk_ctx *plk_tree_context(plk_tree *t);
 int plk_tree_set_model(plk_tree *t, const double *eval, const double *evec, const double *inv_evec,
                        const double *rates, const double *props, int state_unknown, const double *tip_lh);

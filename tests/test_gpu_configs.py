@@ -109,3 +109,44 @@ def test_config4_full_size(prec):
         _, sg = g.partial(0, int(p.nei[0, 0]), safe=True)
         _, so = o.partial(0, int(p.nei[0, 0]), safe=True)
         assert np.array_equal(sg, so)
+
+
+def test_config5_full_size_requests():
+    """Serving at the largest request sizes: a 512-taxon x 100,000-site DNA request (Newton
+    on one branch) and a 512-taxon x 100,000-site protein request (logL), among smaller
+    ones, give the same results alone and in the mix, and their logL matches the oracle."""
+    specs, _ = W.config5_specs(6)
+    big = [(G.SEQ_DNA, 512, 100000, 1, 9001), (G.SEQ_PROTEIN, 512, 100000, 0, 9002)]
+    probs = [W.config5_problem(sp) for sp in big + specs]
+    kinds = [sp[3] for sp in big + specs]
+    say("config5 patterns", [p.nptn for p in probs[:2]])
+
+    def requests():
+        return [G.Request(p, kind=G.REQ_LOGL if k == 0 else G.REQ_OPT_BRANCH, precision=G.FP64,
+                          opt_branch=(p.ntaxa, int(p.nei[p.ntaxa, 0]))) for p, k in zip(probs, kinds)]
+
+    alone = requests()
+    s1 = G.Server(nworkers=1)
+    for r in alone:
+        s1.submit(r)
+        s1.drain()
+    s1.close()
+    mixed = requests()
+    s = G.Server(nworkers=8)
+    for r in mixed:
+        s.submit(r)
+    s.drain()
+    st, cn = s.stats(), s.counters()
+    say("mixed", st, cn)
+    s.close()
+    assert st["failed"] == 0 and cn["batched"] > cn["launches"]
+    for a, m in zip(alone, mixed):
+        ra, rm = a.result(), m.result()
+        assert ra["status"] == 0 and rm["status"] == 0
+        assert (ra["logl_before"], ra["logl_after"], ra["opt_len"]) == (rm["logl_before"], rm["logl_after"], rm["opt_len"])
+    for p, r in zip(probs[:2], mixed[:2]):
+        o = S.Oracle(p, nthreads=16)
+        lo = o.loglik(0, int(p.nei[0, 0]))
+        say("  oracle", p.n, lo, r.result()["logl_before"], rel(r.result()["logl_before"], lo))
+        assert rel(r.result()["logl_before"], lo) <= 1e-10
+        o.close()

@@ -1,5 +1,7 @@
-"""Serving loop: requests of mixed taxa/sites/state counts admitted by HBM footprint
-onto concurrent workers; every result must equal the same request run alone."""
+"""Serving loop: requests of mixed taxa/sites/state counts admitted by HBM footprint,
+their traversals batched into shared launches (continuous batching) and their Newton
+phases overlapping on worker streams; every result must equal the same request run
+alone."""
 import numpy as np
 import pytest
 
@@ -40,10 +42,14 @@ def test_alone_equals_mixed(prec):
         s.submit(r)
     s.drain()
     st = s.stats()
-    say("stats", st)
+    cn = s.counters()
+    say("stats", st, cn)
     s.close()
     assert st["completed"] == 12 and st["failed"] == 0
     assert st["peak_concurrent"] >= 2
+    # continuous batching: traversals of several requests shared launches, trees were reused
+    assert cn["batched"] > cn["launches"]
+    assert cn["pooled_trees"] < 12
     for a, m in zip(alone, mixed):
         ra, rm = a.result(), m.result()
         assert ra["status"] == 0 and rm["status"] == 0
@@ -74,3 +80,33 @@ def test_admission_respects_budget():
     s.drain()
     assert big.result()["status"] == -5
     s.close()
+
+
+@pytest.mark.parametrize("prec", [G.FP64, G.FP32], ids=["fp64", "fp32"])
+def test_tree_reshape_equals_fresh_tree(prec):
+    """A pooled tree reshaped to another problem (larger, then smaller, then a different
+    taxon count) gives exactly the logL and Newton result of a fresh tree."""
+    import ctypes
+    probs = [G.Problem(G.SEQ_PROTEIN, 12, 400, mean_len=0.2, seed=1), G.Problem(G.SEQ_PROTEIN, 30, 3000, seed=2),
+             G.Problem(G.SEQ_PROTEIN, 9, 150, seed=3), G.Problem(G.SEQ_PROTEIN, 40, 900, seed=4)]
+    pooled = G.Tree(probs[0], precision=prec)
+    first = True
+    for p in probs:
+        if not first:
+            G._check(G.lib().plk_tree_reshape(pooled.h, p.ntaxa, p.nptn))
+            pooled.p = p
+            pooled.set_model(p)
+            st = np.ascontiguousarray(p.states, dtype=np.uint8)
+            fr = np.ascontiguousarray(p.freq, dtype=np.float64)
+            G._check(G.lib().plk_tree_set_patterns(pooled.h, st.ctypes.data_as(G._u8p), G._d(fr), None))
+            nei = np.ascontiguousarray(p.nei, dtype=np.int32)
+            bl = np.ascontiguousarray(p.blen, dtype=np.float64)
+            G._check(G.lib().plk_tree_set_topology(pooled.h, nei.ctypes.data_as(G._ip), G._d(bl)))
+        first = False
+        fresh = G.Tree(p, precision=prec)
+        a, b = 0, int(p.nei[0, 0])
+        assert pooled.loglik(a, b) == fresh.loglik(a, b)
+        u = p.ntaxa
+        assert pooled.optimize_one(u, int(p.nei[u, 0])) == fresh.optimize_one(u, int(p.nei[u, 0]))
+        fresh.close()
+    pooled.close()
