@@ -205,6 +205,34 @@ def other_configs(G, no_cpu):
         r["logl_rel_err"] = {k: abs(r[k]["logl"] - ref) / abs(ref) for k in ("fp64", "fp32")}
     log(f"codon 48x5k: fp64 {r['fp64']['ms_per_step']:.2f} ms, fp32 {r['fp32']['ms_per_step']:.2f} ms")
     res["codon_48taxa_5k"] = r
+
+    # serving: 16 requests (16-512 taxa, 1k-100k sites, DNA or protein, logL or Newton on
+    # one branch) submitted at once (the saturated end of the arrival-rate sweep)
+    specs, _ = W.config5_specs(16)
+    probs = [W.config5_problem(sp) for sp in specs]
+    upd = sum((q.ntaxa - 2) * q.nptn for q in probs)
+    r = {"workload": "serving_16_requests_16-512taxa_1k-100k_sites_saturation", "requests": len(probs),
+         "first_traversal_updates": upd}
+    for prec, name in ((G.FP64, "fp64"), (G.FP32, "fp32")):
+        best = None
+        for rep in range(2):  # the first run also fills the tree pool
+            reqs = [G.Request(q, kind=G.REQ_LOGL if sp[3] == 0 else G.REQ_OPT_BRANCH, precision=prec,
+                              opt_branch=(q.ntaxa, int(q.nei[q.ntaxa, 0]))) for q, sp in zip(probs, specs)]
+            srv = G.Server(nworkers=8)
+            for q in reqs:
+                srv.submit(q)
+            srv.drain()
+            st, cn = srv.stats(), srv.counters()
+            srv.close()
+            best = (st, cn) if best is None or st["makespan_s"] < best[0]["makespan_s"] else best
+        st, cn = best
+        r[name] = {"p50_s": st["p50_s"], "p99_s": st["p99_s"], "makespan_s": st["makespan_s"],
+                   "throughput_req_s": len(probs) / st["makespan_s"], "failed": st["failed"],
+                   "shared_launches": cn["launches"], "request_traversals_in_them": cn["batched"],
+                   "peak_concurrent": st["peak_concurrent"]}
+    log(f"serving 16 requests: fp64 p50 {r['fp64']['p50_s'] * 1e3:.1f} ms p99 {r['fp64']['p99_s'] * 1e3:.1f} ms, "
+        f"fp32 p50 {r['fp32']['p50_s'] * 1e3:.1f} ms p99 {r['fp32']['p99_s'] * 1e3:.1f} ms")
+    res["serving"] = r
     return res
 
 
@@ -335,9 +363,15 @@ def main():
                      "roofline_frac": f64["achieved"] / HBM_PEAK_GBS},
             "fp32_over_fp64": f32["ups"] / f64["ups"],
             "gpu_fp32_over_cpu": (f32["ups"] / cpu["value"]) if cpu else None,
+            "latency": "latency_p50_s/latency_p99_s: serving, 16 requests (16-512 taxa, 1k-100k sites) at saturation, "
+                       "FP32 form (_fp64: FP64 form); other_workloads.serving has both",
             "deriv_max_rel_err_vs_oracle": deriv,
             "other_workloads": others,
         },
+        "latency_p50_s": others["serving"]["fp32"]["p50_s"] if others else None,
+        "latency_p99_s": others["serving"]["fp32"]["p99_s"] if others else None,
+        "latency_p50_s_fp64": others["serving"]["fp64"]["p50_s"] if others else None,
+        "latency_p99_s_fp64": others["serving"]["fp64"]["p99_s"] if others else None,
         "value_fp64": f64["ups"],
         "ms_per_step_fp64": f64["ms_per_step"],
         "logl_rel_err_fp64": err64,

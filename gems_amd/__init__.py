@@ -100,6 +100,9 @@ _SIGS = {
     "plk_set_tile_shape": (_int, [_P, _int]),
     "plk_set_level_waves": (_int, [_P, _int]),
     "plk_set_f64_form": (_int, [_P, _int]),
+    "plk_get_pattern_lh_cat": (_int, [_P, _dp]),
+    "plk_get_pattern_scaling": (_int, [_P, _dp, _dp]),
+    "plk_get_pattern_dlogl": (_int, [_P, _dp]),
     "plk_probe_copy": (_int, [ctypes.c_int64, _int, _dp]),
     # model / synthetic data
     "plk_state_unknown": (_int, [_int, _int]),
@@ -445,6 +448,21 @@ class Tree:
 
     def set_tile_shape(self, wpc):
         _check(lib().plk_set_tile_shape(self.context(), wpc))
+
+    def pattern_lh_cat(self):
+        out = np.zeros(self.p.nptn * len(self.p.rates))
+        _check(lib().plk_get_pattern_lh_cat(self.context(), _d(out)))
+        return out.reshape(self.p.nptn, len(self.p.rates))
+
+    def pattern_scaling(self):
+        lh, sc = np.zeros(self.p.nptn), np.zeros(self.p.nptn)
+        _check(lib().plk_get_pattern_scaling(self.context(), _d(lh), _d(sc)))
+        return lh, sc
+
+    def pattern_dlogl(self):
+        out = np.zeros(self.p.nptn)
+        _check(lib().plk_get_pattern_dlogl(self.context(), _d(out)))
+        return out
 
     def set_f64_form(self, form):
         _check(lib().plk_set_f64_form(self.context(), form))

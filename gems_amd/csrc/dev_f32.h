@@ -845,6 +845,7 @@ __device__ inline bool theta_f32_body(const KParams &P, const int tile) {
         const double inv = 1.0 / lh;
         const double dfr = d1 * inv, ddfr = d2 * inv;
         const double t1 = dfr * f, t2 = ddfr * f;
+        if (P.g_frac && p < P.nptn) P.g_frac[p] = dfr;  // G_matrix row
         r0 = wave_sum(p < P.nptn ? t1 : 0.0);
         r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
         if (L3) r2 = wave_sum(p < P.nptn ? (log(lh) + e * kLn2) * f : 0.0);

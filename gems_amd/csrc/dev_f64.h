@@ -662,9 +662,18 @@ __device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
         lh = fabs(lh) + invar;
         const double f = (p < P.nptn) ? P.freq[p] : 0.0;
         if (MODE == MODE_LOGL) {
-            // 3096-3108
+            // 3096-3108; side outputs _pattern_lh_cat (3023, SAFE-adjusted 3076-3083) and
+            // the save_log_value = false scaling (3103-3104)
             const double l = log(lh) + sc * kLogST;
-            if (p < P.nptn) P.ptn_lh[p] = l;
+            if (p < P.nptn) {
+                P.ptn_lh[p] = l;
+                if (P.ptn_lh_cat) {
+                    for (int c = 0; c < ncat; c++)
+                        P.ptn_lh_cat[p * ncat + c] = P.safe ? s_cat[0][c][lane] * safe_factor(s_sc[c][lane] - sc_min)
+                                                            : s_cat[0][c][lane];
+                    P.ptn_scaling[p] = sc * kLogST;
+                }
+            }
             r0 = wave_sum(p < P.nptn ? l * f : 0.0);
         } else {
             // buffer_scale_all (2178-2179, 2231-2232) and df/ddf (2504-2515)
@@ -672,6 +681,7 @@ __device__ inline bool traverse_f64_body(const KParams &P, const int tile) {
             const double inv = 1.0 / lh;
             const double dfr = d1 * inv, ddfr = d2 * inv;
             const double t1 = dfr * f, t2 = ddfr * f;
+            if (P.g_frac && p < P.nptn) P.g_frac[p] = dfr;  // G_matrix row (2518)
             r0 = wave_sum(p < P.nptn ? t1 : 0.0);
             r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
             if (L3) r2 = wave_sum(p < P.nptn ? (log(lh) + sc * kLogST) * f : 0.0);
@@ -733,6 +743,7 @@ __device__ inline bool theta_eval_f64_body(const KParams &P, const int tile, con
             const double inv = 1.0 / lh;
             const double dfr = d1 * inv, ddfr = d2 * inv;
             const double t1 = dfr * f, t2 = ddfr * f;
+            if (P.g_frac && p < P.nptn) P.g_frac[p] = dfr;  // G_matrix row (2518)
             r0 = wave_sum(p < P.nptn ? t1 : 0.0);
             r1 = wave_sum(p < P.nptn ? t2 - t1 * dfr : 0.0);
         } else {

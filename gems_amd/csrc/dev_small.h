@@ -472,6 +472,20 @@ __device__ inline void traverse_small_body(const KParams &P, const int tile, cha
         // unscaled there; the FP32 exponent is folded in by log_lh_expo.
         const double l = FORM == 0 ? log(fabs(lh) + invar) + sc : log_lh_expo(lh, se, invar);
         if (p < P.nptn) P.ptn_lh[p] = l;
+        if (FORM == 0 && P.ptn_lh_cat && p < P.nptn) {
+            // _pattern_lh_cat (tree/phylokernelnew.h:3023, SAFE-adjusted 3076-3083) and the
+            // save_log_value = false scaling (3103-3104)
+            int mn = 0;
+            if (P.safe) {
+                mn = s_sc[0][lane];
+                for (int c = 1; c < ncat; c++) mn = min(mn, s_sc[c][lane]);
+            }
+            for (int c = 0; c < ncat; c++) {
+                const int d = P.safe ? s_sc[c][lane] - mn : 0;
+                P.ptn_lh_cat[p * ncat + c] = s_cat[c][lane] * (d == 0 ? 1.0 : (d == 1 ? kST : 0.0));
+            }
+            P.ptn_scaling[p] = sc;
+        }
         r0 = wave_sum(p < P.nptn ? l * P.freq[p] : 0.0);
     }
     finish_tiles(P, r0, 0.0, tile, P.ntiles);

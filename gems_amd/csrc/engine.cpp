@@ -115,6 +115,9 @@ KParams make_params(Ctx &c) {
     p.ops = c.d_ops;
     p.nops = 0;
     p.ptn_lh = c.d_ptn_lh;
+    p.ptn_lh_cat = c.d_side;
+    p.ptn_scaling = c.d_side + (size_t)c.nptn_pad * c.ncat;
+    p.g_frac = c.d_side + (size_t)c.nptn_pad * (c.ncat + 1);
     p.theta = c.d_theta;
     p.buf_scale = c.d_buf_scale;
     p.tile_sums = c.d_tile_sums;
@@ -551,6 +554,8 @@ extern "C" int plk_create(plk_ctx **out, int nstates, int ncat, int ntaxa, int n
     CREATE_TRY(hipMallocAsync((void **)&c->d_counter, sizeof(unsigned), c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_sink, kSinkBytes, c->stream));
     CREATE_TRY(hipMallocAsync((void **)&c->d_newton, sizeof(NewtonState), c->stream));
+    c->cap_side = sizeof(double) * (size_t)c->nptn_pad * (ncat + 2);
+    CREATE_TRY(hipMallocAsync((void **)&c->d_side, c->cap_side, c->stream));
     c->cap_part = rs * slot * nslots;
     c->cap_scale = 2 * scale_slot * nslots;
     c->cap_states = (size_t)ntaxa * c->nptn_pad;
@@ -594,7 +599,7 @@ extern "C" int plk_destroy(plk_ctx *c) {
     void *bufs[] = {c->d_part, c->d_scale, c->d_states, c->d_freq, c->d_invar, c->d_mats, c->d_model,
                     c->d_ops, c->d_jobs, c->d_theta, c->d_buf_scale, c->d_ptn_lh, c->d_tile_sums,
                     c->d_result, c->d_counter, c->d_mparams, c->d_mitems, c->d_mtile, c->d_mres,
-                    c->d_mcount, c->d_newton, c->d_mnewton, c->d_sink, c->d_uinv_frag};
+                    c->d_mcount, c->d_newton, c->d_mnewton, c->d_sink, c->d_uinv_frag, c->d_side};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, c->stream);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -642,6 +647,7 @@ extern "C" int plk_reshape(plk_ctx *c, int ntaxa, int nptn, int nslots) {
     }
     if (!rc) rc = grow(*c, (void **)&c->d_theta, c->cap_theta, sizeof(double) * slot);
     if (!rc) rc = grow(*c, (void **)&c->d_tile_sums, c->cap_tiles, sizeof(double) * 2 * 4 * ntiles);
+    if (!rc) rc = grow(*c, (void **)&c->d_side, c->cap_side, sizeof(double) * (size_t)pad * (c->ncat + 2));
     if (rc) return rc;
     if (ntiles != c->ntiles && c->mcap) {
         // the many-branch buffers hold (branch, tile) items: rebuilt on next use
@@ -1046,6 +1052,7 @@ extern "C" int plk_newton_multi(plk_ctx *c, int nbr, const int *a, const int *b,
     base.nops = 0;
     base.theta = nullptr;
     base.buf_scale = nullptr;
+    base.g_frac = nullptr;  // per-pattern derivative output: single-branch calls only
     std::vector<KParams> ps(nbr, base);
     NewtonState *h = c->h_mnewton;
     for (int i = 0; i < nbr; i++) {
@@ -1206,6 +1213,39 @@ extern "C" int plk_set_tile_shape(plk_ctx *c, int waves_per_category) {
     return PLK_OK;
 }
 
+static int copy_side(plk_ctx *c, size_t off, size_t count, double *out) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(out, c->d_side + off, sizeof(double) * count, hipMemcpyDeviceToHost));
+    return PLK_OK;
+}
+
+extern "C" int plk_get_pattern_lh_cat(plk_ctx *c, double *out) {
+    clear_error();
+    if (!c || !out) return fail(PLK_ERR_ARG, "plk_get_pattern_lh_cat: null argument");
+    if (c->prec != PLK_FP64) return fail(PLK_ERR_STATE, "plk_get_pattern_lh_cat: FP64 form only");
+    if (!c->ptn_lh_valid) return fail(PLK_ERR_STATE, "plk_get_pattern_lh_cat: no logL evaluated yet");
+    return copy_side(c, 0, (size_t)c->nptn * c->ncat, out);  // [pattern][category]
+}
+
+extern "C" int plk_get_pattern_scaling(plk_ctx *c, double *lh, double *scaling) {
+    clear_error();
+    if (!c || !lh || !scaling) return fail(PLK_ERR_ARG, "plk_get_pattern_scaling: null argument");
+    if (c->prec != PLK_FP64) return fail(PLK_ERR_STATE, "plk_get_pattern_scaling: FP64 form only");
+    if (!c->ptn_lh_valid) return fail(PLK_ERR_STATE, "plk_get_pattern_scaling: no logL evaluated yet");
+    int rc = copy_side(c, (size_t)c->nptn_pad * c->ncat, c->nptn, scaling);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(lh, c->d_ptn_lh, sizeof(double) * c->nptn, hipMemcpyDeviceToHost));
+    // lh = |sum| + invar, recovered from the stored log (one rounding of exp(log))
+    for (int p = 0; p < c->nptn; p++) lh[p] = std::exp(lh[p] - scaling[p]);
+    return PLK_OK;
+}
+
+extern "C" int plk_get_pattern_dlogl(plk_ctx *c, double *out) {
+    clear_error();
+    if (!c || !out) return fail(PLK_ERR_ARG, "plk_get_pattern_dlogl: null argument");
+    return copy_side(c, (size_t)c->nptn_pad * (c->ncat + 1), c->nptn, out);
+}
+
 extern "C" int plk_set_f64_form(plk_ctx *c, int form) {
     if (!c || form < -1 || form > 1) return fail(PLK_ERR_ARG, "plk_set_f64_form: form must be -1, 0 or 1");
     c->f64_form = form;
@@ -1278,6 +1318,7 @@ extern "C" int plk_branch_derivs_multi(plk_ctx *c, int nbr, const int *a, const 
     base.nops = 0;
     base.theta = nullptr;
     base.buf_scale = nullptr;
+    base.g_frac = nullptr;  // per-pattern derivative output: single-branch calls only
     std::vector<KParams> ps(nbr, base);
     for (int i = 0; i < nbr; i++) {
         ps[i].br_a = ra[i];

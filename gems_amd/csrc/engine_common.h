@@ -91,6 +91,9 @@ struct KParams {
     int br_a, br_b;           // branch stage: node refs (br_a may be a tip), br_b a slot
     double br_len;
     double *ptn_lh;           // nptn_pad
+    double *ptn_lh_cat;       // nptn_pad x ncat: per-category likelihood of the last logL (FP64 forms)
+    double *ptn_scaling;      // nptn_pad: the log scaling of each pattern's likelihood (FP64 forms)
+    double *g_frac;           // nptn_pad: L'/L per pattern of the last derivative evaluation (or null)
     void *theta;              // nptn_pad x block reals
     double *buf_scale;        // nptn_pad
     double *tile_sums;        // ntiles x 2
@@ -151,6 +154,8 @@ struct Ctx {
     int ops_cap = 0;
     void *d_theta = nullptr;
     double *d_buf_scale = nullptr, *d_ptn_lh = nullptr, *d_tile_sums = nullptr, *d_result = nullptr;
+    double *d_side = nullptr;  // per-pattern side outputs: lh_cat (ncat per pattern) | scaling | L'/L
+    size_t cap_side = 0;
     unsigned *d_counter = nullptr;
     float *d_sink = nullptr;
     int ncu = 0;  // compute units of the device

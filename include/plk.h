@@ -176,6 +176,20 @@ int plk_set_tile_shape(plk_ctx *ctx, int waves_per_category);
  * patterns); form 0 the vector-ALU kernels; -1 (default) picks by state count, the faster
  * one measured (61 states: matrix cores, 20: vector ALU).  Same results to rounding (the
  * sums run in a different order); scale counts follow the same rule. */
+/* Side outputs of the last plk_branch_loglik (FP64 form), as computeLikelihoodBranch
+ * leaves them: lh_cat[p * ncat + c] = the category's likelihood w_c * sum_i val * node * dad
+ * in the pattern's scaled frame, SAFE categories above the minimum count scaled by 2^-256
+ * or zeroed (_pattern_lh_cat, tree/phylokernelnew.h:3023, 3063-3088); nptn x ncat doubles. */
+int plk_get_pattern_lh_cat(plk_ctx *ctx, double *lh_cat);
+/* The save_log_value = false form of the last plk_branch_loglik (FP64 form): lh[p] =
+ * |sum_c lh_cat| + ptn_invar and scaling[p] = the pattern's log scaling factor, so that
+ * logL_p = log(lh[p]) + scaling[p] (_pattern_lh / _pattern_scaling,
+ * tree/phylokernelnew.h:3099-3107); nptn doubles each. */
+int plk_get_pattern_scaling(plk_ctx *ctx, double *lh, double *scaling);
+/* L'_p / L_p of every pattern at the last single-branch derivative evaluation
+ * (plk_branch_derivs, plk_newton_branch): the row of G_matrix (tree/phylokernelnew.h:2518)
+ * for that branch; nptn doubles. */
+int plk_get_pattern_dlogl(plk_ctx *ctx, double *dlogl);
 int plk_set_f64_form(plk_ctx *ctx, int form);
 
 /* 4-state models: 1..4 = level-parallel kernel: the ops are sorted into dependency

@@ -537,6 +537,44 @@ def pruning_loglik(prob, a, b, dtype=np.float64):
     return lp, float(np.dot(lp, prob.freq))
 
 
+def pruning_lh_cat(prob, a, b):
+    """True per-category pattern likelihoods w_c L_c(p) (ncat x nptn) at branch (a, b) by
+    the same independent real-space pruning as pruning_loglik (without ptn_invar)."""
+    ncat = len(prob.rates)
+    P = {}
+
+    def pmat(t, c):
+        key = (round(t, 17), c)
+        if key not in P:
+            P[key] = scipy.linalg.expm(prob.Q * prob.rates[c] * t)
+        return P[key]
+
+    def down(node, dad):
+        if node < prob.ntaxa:
+            T = np.stack([tip_likelihood(prob, s) for s in prob.states[node]])
+            return np.repeat(T[None], ncat, axis=0), np.zeros(prob.nptn)
+        acc = np.ones((ncat, prob.nptn, prob.n))
+        lsc = np.zeros(prob.nptn)
+        for k in range(3):
+            ch = prob.nei[node, k]
+            if ch < 0 or ch == dad:
+                continue
+            Lc, sc = down(ch, node)
+            for c in range(ncat):
+                acc[c] *= Lc[c] @ pmat(prob.blen[node, k], c).T
+            lsc += sc
+        m = acc.max(axis=(0, 2))
+        m[m == 0] = 1.0
+        return acc / m[None, :, None], lsc + np.log(m)
+
+    La, sa = down(a, b)
+    Lb, sb = down(b, a)
+    t = prob.blen[a, list(prob.nei[a]).index(b)]
+    cat = np.stack([prob.props[c] * np.einsum("px,px->p", La[c] * prob.pi[None, :], Lb[c] @ pmat(t, c).T)
+                    for c in range(ncat)])
+    return cat * np.exp(sa + sb)[None, :]
+
+
 def ptn_invar_identity(prob, p_inv):
     """Invariant-site mass of each pattern from its definition: p_inv times the probability
     that every taxon shows one common unchanged state, i.e. p_inv * sum_x pi_x * prod_t

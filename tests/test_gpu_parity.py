@@ -310,3 +310,38 @@ def test_scale_counts_follow_rule(case):
             top = max(top, int(sg.max()))
             checked += 1
     assert checked >= 3 * (T - 2) // 7 and top >= 2
+
+
+@pytest.mark.parametrize("safe", [0, 1], ids=["norm", "safe"])
+@pytest.mark.parametrize("seq", [G.SEQ_DNA, G.SEQ_PROTEIN], ids=["dna", "protein"])
+def test_pattern_side_outputs(seq, safe):
+    """_pattern_lh_cat times the pattern's scaling equals the true per-category likelihood
+    of an independent real-space pruning (1e-10 of the pattern's likelihood); the
+    save_log_value = false form gives
+    log(lh) + scaling = the per-pattern logL; L'/L per pattern (the G_matrix row) equals
+    a central finite difference of the per-pattern logL and sums, weighted, to df."""
+    p = G.Problem(seq, 10, 400, mean_len=0.2, alpha=0.6, seed=21)
+    g = G.Tree(p, scaling=safe)
+    a, b = 0, int(p.nei[0, 0])
+    lg, plg = g.loglik(a, b, with_patterns=True)
+    cat = g.pattern_lh_cat()
+    lh, sc = g.pattern_scaling()
+    np.testing.assert_allclose(np.log(lh) + sc, plg, rtol=1e-13, atol=1e-12)
+    ref = S.pruning_lh_cat(p, a, b)  # ncat x nptn
+    # relative to the pattern's likelihood: a category far below the others is an
+    # eigen-space sum with cancellation, exact only to ~1e-16 of the pattern total
+    err = np.abs((cat * np.exp(sc)[:, None]).T - ref) / ref.sum(axis=0)[None, :]
+    assert float(err.max()) <= 1e-10
+    u, v = int(p.nei[p.ntaxa, 0]), p.ntaxa
+    df, _, _ = g.derivs(u, v)
+    dl = g.pattern_dlogl()
+    assert abs(float(np.dot(dl, p.freq)) - df) <= 1e-9 * max(1.0, abs(df))
+    t0 = g.get_length(u, v)
+    h = 1e-5 * max(t0, 1e-3)
+    o = S.Oracle(p, safe=bool(safe))
+    o.set_length(u, v, t0 + h)
+    _, lp = o.loglik(u, v, with_patterns=True)
+    o.set_length(u, v, t0 - h)
+    _, lm = o.loglik(u, v, with_patterns=True)
+    fd = (lp - lm) / (2 * h)
+    np.testing.assert_allclose(dl, fd, rtol=1e-5, atol=1e-6)
