@@ -733,9 +733,21 @@ extern "C" int plk_set_model(plk_ctx *c, const double *eval, const double *evec,
             for (int i = 0; i < n; i++) v += evec[x * n + i] * tip_lh[s * n + i];
             tipvec[(size_t)s * n + x] = v;
         }
-    int i0 = 0;
-    for (int i = 1; i < n; i++)
-        if (std::fabs(eval[i]) < std::fabs(eval[i0])) i0 = i;
+    // the stationary row: among the (near-)zero eigenvalues, the U^-1 row with the most
+    // non-zero entries (states of zero frequency also have eigenvalue 0, with a unit row)
+    double emax = 1.0;
+    for (int i = 0; i < n; i++) emax = std::max(emax, std::fabs(eval[i]));
+    int i0 = -1, best_nz = -1;
+    for (int i = 0; i < n; i++) {
+        if (std::fabs(eval[i]) > 1e-8 * emax) continue;
+        int nzc = 0;
+        for (int x = 0; x < n; x++) nzc += inv_evec[i * n + x] != 0.0;
+        if (nzc > best_nz || (nzc == best_nz && std::fabs(eval[i]) < std::fabs(eval[i0]))) {
+            best_nz = nzc;
+            i0 = i;
+        }
+    }
+    if (i0 < 0) return fail(PLK_ERR_ARG, "plk_set_model: no zero eigenvalue");
     double psum = 0.0;
     for (int x = 0; x < n; x++) psum += inv_evec[i0 * n + x];
     if (!(std::fabs(psum) > 0.0)) return fail(PLK_ERR_ARG, "plk_set_model: no stationary eigenvector");

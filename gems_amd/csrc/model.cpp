@@ -23,9 +23,12 @@ extern "C" int plk_rate_matrix(int n, const double *exch, const double *pi, doub
     std::vector<double> R((size_t)n * n, 0.0);
     for (int i = 0, k = 0; i < n; i++)
         for (int j = i + 1; j < n; j++, k++) R[i * n + j] = R[j * n + i] = exch[k];
+    // zero frequencies are allowed: such a state's column is zero (no rate into it); its
+    // row does not enter the normalisation and plk_eigen leaves it out
+    // (model/modelmarkov.cpp:1453-1500)
     double norm = 0.0;
     for (int i = 0; i < n; i++) {
-        if (!(pi[i] > 0.0)) return plk::fail(PLK_ERR_ARG, "plk_rate_matrix: state frequencies must be > 0");
+        if (!(pi[i] >= 0.0)) return plk::fail(PLK_ERR_ARG, "plk_rate_matrix: state frequencies must be >= 0");
         double rs = 0.0;
         for (int j = 0; j < n; j++) {
             double q = (i == j) ? 0.0 : R[i * n + j] * pi[j];
@@ -83,9 +86,62 @@ static bool jacobi_eigen(int n, std::vector<double> &A, std::vector<double> &w, 
     return true;
 }
 
+static int eigen_full(int n, const double *Q, const double *pi, double *eval, double *evec, double *inv_evec);
+
+constexpr double kZeroFreq = 1e-10;  // ZERO_FREQ (utils/eigendecomposition.h:24)
+
 extern "C" int plk_eigen(int n, const double *Q, const double *pi, double *eval, double *evec, double *inv_evec) {
     if (n < 2 || n > PLK_MAX_STATES || !Q || !pi || !eval || !evec || !inv_evec)
         return plk::fail(PLK_ERR_ARG, "plk_eigen: bad arguments");
+    std::vector<int> nz;
+    for (int i = 0; i < n; i++) {
+        if (!(pi[i] >= 0.0)) return plk::fail(PLK_ERR_ARG, "plk_eigen: state frequencies must be >= 0");
+        if (pi[i] > kZeroFreq) nz.push_back(i);
+    }
+    const int m = (int)nz.size();
+    if (m == n) return eigen_full(n, Q, pi, eval, evec, inv_evec);
+    if (m < 2) return plk::fail(PLK_ERR_ARG, "plk_eigen: fewer than two states with non-zero frequency");
+    // States with frequency <= ZERO_FREQ (model/modelmarkov.cpp:1453-1500, 1563-1597): the
+    // system of the other states, their frequencies renormalised and their rows of Q
+    // rescaled to one substitution per unit time, is decomposed alone; eigenvalue k of it
+    // goes to the k-th non-zero state's position, its eigenvectors fill the non-zero rows
+    // and columns, and a zero-frequency state gets eigenvalue 0 and unit vectors.
+    std::vector<double> qs((size_t)m * m), ps(m), ev(m), U((size_t)m * m), Ui((size_t)m * m);
+    double psum = 0.0;
+    for (int a = 0; a < m; a++) psum += pi[nz[a]];
+    for (int a = 0; a < m; a++) ps[a] = pi[nz[a]] / psum;
+    double norm = 0.0;
+    for (int a = 0; a < m; a++) {
+        double rs = 0.0;
+        for (int b = 0; b < m; b++) {
+            qs[a * m + b] = a == b ? 0.0 : Q[nz[a] * n + nz[b]];
+            rs += qs[a * m + b];
+        }
+        qs[a * m + a] = -rs;
+        norm += ps[a] * rs;
+    }
+    if (!(norm > 0.0)) return plk::fail(PLK_ERR_ARG, "plk_eigen: zero total rate among the non-zero states");
+    for (double &v : qs) v /= norm;
+    int rc = eigen_full(m, qs.data(), ps.data(), ev.data(), U.data(), Ui.data());
+    if (rc) return rc;
+    for (int i = 0; i < n * n; i++) evec[i] = inv_evec[i] = 0.0;
+    for (int i = 0; i < n; i++) {
+        eval[i] = 0.0;
+        evec[i * n + i] = inv_evec[i * n + i] = 1.0;
+    }
+    for (int a = 0; a < m; a++) {
+        const int i = nz[a];
+        eval[i] = std::isnan(ev[a]) ? 0.0 : ev[a];
+        evec[i * n + i] = inv_evec[i * n + i] = 0.0;
+        for (int b = 0; b < m; b++) {
+            evec[i * n + nz[b]] = U[a * m + b];
+            inv_evec[i * n + nz[b]] = Ui[a * m + b];
+        }
+    }
+    return PLK_OK;
+}
+
+static int eigen_full(int n, const double *Q, const double *pi, double *eval, double *evec, double *inv_evec) {
     // S = Pi^1/2 Q Pi^-1/2 (model/modelmarkov.cpp:1520), symmetric for reversible Q
     std::vector<double> s(n), S((size_t)n * n), w, Vm;
     for (int i = 0; i < n; i++) {

@@ -70,7 +70,9 @@ def test_fp32_derivatives(case):
         e1 = abs(dg - do) / max(1.0, abs(do))
         e2 = abs(ddg - ddo) / max(1.0, abs(ddo))
         say(f"  branch {u}-{v}: df {dg:.6f} vs {do:.6f} ({e1:.1e}), ddf {ddg:.4f} vs {ddo:.4f} ({e2:.1e})")
-        assert e1 < 1e-3 and e2 < 1e-3
+        # FP32 partials (7 digits) carried into an FP64 derivative sum: measured up to
+        # ~1.2e-5 relative (DNA, 32 taxa x 10k sites), ~7e-7 (protein); bound ~10x that
+        assert e1 < 2e-4 and e2 < 2e-4
         assert rel(g.loglik_from_theta(u, v), o.loglik_from_theta(u, v)) <= LOGL_TOL
 
 

@@ -8,6 +8,8 @@ import numpy as np
 import pytest
 
 import gems_amd as G
+import scipy.linalg
+import support as S
 
 FIX = json.loads((pathlib.Path(__file__).parent / "golden" / "model_fixtures.json").read_text())
 L = G.lib()
@@ -132,3 +134,36 @@ def test_simulator(seq, T, sites):
     q = G.Problem(seq, T, sites, seed=5)
     assert np.array_equal(p.states, q.states) and np.array_equal(p.blen, q.blen)
     assert abs(p.pi.sum() - 1) < 1e-12
+
+
+def _lib_eigen(Q, pi):
+    n = len(pi)
+    ev, U, Ui = np.zeros(n), np.zeros(n * n), np.zeros(n * n)
+    Qc = np.ascontiguousarray(Q, dtype=np.float64).ravel()
+    pic = np.ascontiguousarray(pi, dtype=np.float64)
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    assert G.lib().plk_eigen(n, dp(Qc), dp(pic), dp(ev), dp(U), dp(Ui)) == 0, G.lib().plk_last_error()
+    return ev, U.reshape(n, n), Ui.reshape(n, n)
+
+
+@pytest.mark.parametrize("n,zeros", [(4, [2]), (20, [3, 17]), (61, [0, 30, 60])])
+def test_zero_frequency_states(n, zeros):
+    """States of zero frequency (model/modelmarkov.cpp:1453-1500, 1563-1597): the other
+    states' system is decomposed alone, the zero states get eigenvalue 0 and unit vectors.
+    Then U exp(L t) U^-1 is expm(Q t) on the non-zero states (scipy), the identity on the
+    zero ones, and no non-zero state moves into a zero one."""
+    rng = np.random.default_rng(n)
+    pi = rng.dirichlet(np.ones(n))
+    pi[zeros] = 0.0
+    pi /= pi.sum()
+    exch = S.codon_exch(2.0, 0.5)[0] if n == 61 else rng.uniform(0.5, 2.0, n * (n - 1) // 2)
+    Q = S.rate_matrix(exch, pi)
+    ev, U, Ui = _lib_eigen(Q, pi)
+    nz = [i for i in range(n) if i not in zeros]
+    Qn = Q[np.ix_(nz, nz)]
+    for t in (0.05, 0.7):
+        P = U @ np.diag(np.exp(ev * t)) @ Ui
+        np.testing.assert_allclose(P[np.ix_(nz, nz)], scipy.linalg.expm(Qn * t), atol=1e-12)
+        np.testing.assert_allclose(P[np.ix_(zeros, range(n))], np.eye(n)[zeros], atol=1e-14)
+        np.testing.assert_allclose(P[np.ix_(nz, zeros)], 0.0, atol=1e-14)
+    assert all(ev[z] == 0.0 for z in zeros)
