@@ -87,6 +87,25 @@ def time_oracle(p, threads, seconds, min_reps=2, max_reps=50, safe=False):
     return ref, float(np.median(times)), len(times)
 
 
+def cpu_leg(workload, threads, seconds, max_reps):
+    """Time the oracle in a child process that never touches the GPU (so no GPU runtime
+    thread shares a core with the oracle's pinned OpenMP team): (reference logL, median s, reps)."""
+    import subprocess
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", workload, "--threads", str(threads),
+                          "--seconds", str(seconds), "--max-reps", str(max_reps)],
+                         check=True, capture_output=True, text=True, env=dict(os.environ, OMP_PROC_BIND="close"))
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return r["ref"], r["median_s"], r["reps"]
+
+
+def cpu_leg_main(args):
+    from gems_amd import workloads as W
+    p = {"protein": W.config2, "dna": W.config1, "codon": W.config4}[args.cpu_leg]()
+    ref, med, n = time_oracle(p, args.threads, args.seconds, max_reps=args.max_reps,
+                              min_reps=1 if args.threads == 1 else 2, safe=(p.n == 61))
+    print(json.dumps({"ref": ref, "median_s": med, "reps": n}), flush=True)
+
+
 def cpu_baseline(p, seconds=10.0):
     """The oracle (FP64 restatement of the reference kernel, pattern-interleaved SIMD lanes,
     OpenMP packets) on every host core, threads bound close, and on one core."""
@@ -94,8 +113,8 @@ def cpu_baseline(p, seconds=10.0):
     import support as S
 
     cores = host_cores()
-    ref, med, n = time_oracle(p, cores, seconds)
-    _, med1, n1 = time_oracle(p, 1, seconds / 2, min_reps=1, max_reps=3)
+    ref, med, n = cpu_leg("protein", cores, seconds, 50)
+    _, med1, n1 = cpu_leg("protein", 1, seconds / 2, 3)
     upd = (p.ntaxa - 2) * p.nptn
     return ref, {
         "value": upd / med,
@@ -104,7 +123,7 @@ def cpu_baseline(p, seconds=10.0):
         "kind": "port",
         "sample": f"{n} full traversals + branch logL of the bench problem (median), "
                   f"{S.oracle_lib().orc_lanes()}-lane SIMD oracle, OpenMP packets, OMP_PROC_BIND=close, "
-                  f"all {cores} cores of this host",
+                  f"all {cores} cores of this host, in a child process that does not use the GPU",
         "value_1core": upd / med1,
         "sample_1core": f"{n1} traversals on one core (median)",
         "cpu_model": cpu_model(),
@@ -151,7 +170,7 @@ def other_configs(G, no_cpu):
                                 "peak": HBM_PEAK_GBS, "frac": upd * bpu / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     r["fp32_over_fp64"] = r["fp32"]["updates_per_s"] / r["fp64"]["updates_per_s"]
     if not no_cpu:
-        ref, med, n = time_oracle(p, host_cores(), 4.0)
+        ref, med, n = cpu_leg("dna", host_cores(), 4.0, 50)
         r["cpu_updates_per_s"] = upd / med
         r["cpu_cores"] = host_cores()
         r["fp32_over_cpu"] = r["fp32"]["updates_per_s"] / r["cpu_updates_per_s"]
@@ -199,7 +218,7 @@ def other_configs(G, no_cpu):
                    "roofline": {"bound": "mfma", "unit": "TFLOP/s", "achieved": tf, "peak": pk, "frac": tf / pk}}
     r["fp32_over_fp64"] = r["fp32"]["updates_per_s"] / r["fp64"]["updates_per_s"]
     if not no_cpu:
-        ref, med, n = time_oracle(p, host_cores(), 4.0, max_reps=3, safe=True)
+        ref, med, n = cpu_leg("codon", host_cores(), 4.0, 3)
         r["cpu_updates_per_s"] = upd / med
         r["cpu_cores"] = host_cores()
         r["logl_rel_err"] = {k: abs(r[k]["logl"] - ref) / abs(ref) for k in ("fp64", "fp32")}
@@ -245,7 +264,14 @@ def main():
     ap.add_argument("--sites", type=int, default=50000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-configs", action="store_true", help="skip the compact figures of the other workloads")
+    ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)  # child process: time the oracle only
+    ap.add_argument("--threads", type=int, default=1, help=argparse.SUPPRESS)
+    ap.add_argument("--seconds", type=float, default=4.0, help=argparse.SUPPRESS)
+    ap.add_argument("--max-reps", type=int, default=50, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg:
+        cpu_leg_main(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "the pattern-sharded multi-GPU path is not part of this build", file=sys.stderr)
