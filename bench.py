@@ -98,6 +98,19 @@ def cpu_leg(workload, threads, seconds, max_reps):
     return r["ref"], r["median_s"], r["reps"]
 
 
+CPU_LEGS = {}  # workload -> (reference logL, median s, reps), measured before any GPU or oracle use here
+
+
+def run_cpu_legs(seconds=10.0):
+    """All CPU-baseline timings, first thing in the run: each in its own child process, while
+    this process has neither a GPU runtime nor an OpenMP team of its own running."""
+    cores = host_cores()
+    CPU_LEGS["protein"] = cpu_leg("protein", cores, seconds, 50)
+    CPU_LEGS["protein_1core"] = cpu_leg("protein", 1, seconds / 2, 3)
+    CPU_LEGS["dna"] = cpu_leg("dna", cores, 4.0, 50)
+    CPU_LEGS["codon"] = cpu_leg("codon", cores, 4.0, 3)
+
+
 def cpu_leg_main(args):
     from gems_amd import workloads as W
     p = {"protein": W.config2, "dna": W.config1, "codon": W.config4}[args.cpu_leg]()
@@ -113,8 +126,8 @@ def cpu_baseline(p, seconds=10.0):
     import support as S
 
     cores = host_cores()
-    ref, med, n = cpu_leg("protein", cores, seconds, 50)
-    _, med1, n1 = cpu_leg("protein", 1, seconds / 2, 3)
+    ref, med, n = CPU_LEGS["protein"]
+    _, med1, n1 = CPU_LEGS["protein_1core"]
     upd = (p.ntaxa - 2) * p.nptn
     return ref, {
         "value": upd / med,
@@ -170,7 +183,7 @@ def other_configs(G, no_cpu):
                                 "peak": HBM_PEAK_GBS, "frac": upd * bpu / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
     r["fp32_over_fp64"] = r["fp32"]["updates_per_s"] / r["fp64"]["updates_per_s"]
     if not no_cpu:
-        ref, med, n = cpu_leg("dna", host_cores(), 4.0, 50)
+        ref, med, n = CPU_LEGS["dna"]
         r["cpu_updates_per_s"] = upd / med
         r["cpu_cores"] = host_cores()
         r["fp32_over_cpu"] = r["fp32"]["updates_per_s"] / r["cpu_updates_per_s"]
@@ -218,7 +231,7 @@ def other_configs(G, no_cpu):
                    "roofline": {"bound": "mfma", "unit": "TFLOP/s", "achieved": tf, "peak": pk, "frac": tf / pk}}
     r["fp32_over_fp64"] = r["fp32"]["updates_per_s"] / r["fp64"]["updates_per_s"]
     if not no_cpu:
-        ref, med, n = cpu_leg("codon", host_cores(), 4.0, 3)
+        ref, med, n = CPU_LEGS["codon"]
         r["cpu_updates_per_s"] = upd / med
         r["cpu_cores"] = host_cores()
         r["logl_rel_err"] = {k: abs(r[k]["logl"] - ref) / abs(ref) for k in ("fp64", "fp32")}
@@ -276,6 +289,10 @@ def main():
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
               "the pattern-sharded multi-GPU path is not part of this build", file=sys.stderr)
         sys.exit(2)
+
+    if not args.no_cpu:
+        run_cpu_legs()
+        log("cpu legs: " + ", ".join(f"{k} {v[1] * 1e3:.2f} ms" for k, v in CPU_LEGS.items()))
 
     import gems_amd as G
 
