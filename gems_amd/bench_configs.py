@@ -172,6 +172,63 @@ def config1(quick):
             "fp32_over_cpu": r32["updates_per_s"] / cpu}
 
 
+def _topology(ntaxa, shape):
+    """Unrooted binary topology as plk_sim_create_on_tree takes it ((2T-2) x 3, leaves
+    0..T-1 with their neighbour in slot 0): 'caterpillar' (T-2 dependent levels from the
+    first leaf's branch) or 'balanced' (about log2 T levels)."""
+    nn = 2 * ntaxa - 2
+    nei = -np.ones((nn, 3), dtype=np.int32)
+
+    def link(a, b):
+        ka = list(nei[a]).index(-1)
+        kb = list(nei[b]).index(-1)
+        nei[a, ka], nei[b, kb] = b, a
+
+    internals = list(range(ntaxa, nn))
+    if shape == "caterpillar":
+        link(internals[0], 0)
+        link(internals[0], 1)
+        for i in range(len(internals) - 1):
+            link(internals[i], internals[i + 1])
+            link(internals[i + 1], i + 2)
+        link(internals[-1], ntaxa - 1)
+    else:
+        # pair leaves, then pair subtrees, until three remain for the last internal node
+        nodes, nxt = list(range(ntaxa)), 0
+        while len(nodes) > 3:
+            a, b = nodes.pop(0), nodes.pop(0)
+            v = internals[nxt]
+            nxt += 1
+            link(v, a)
+            link(v, b)
+            nodes.append(v)
+        v = internals[nxt]
+        for x in nodes:
+            link(v, x)
+    return nei
+
+
+def config1_levels(quick):
+    """Where a DNA traversal's time goes: the same 30 ops x 6,002 patterns (32 taxa, 10k
+    sites) on trees whose post-order has 5, 13 (the Yule tree of config 1) or 30 dependent
+    levels, and the Yule tree at 2k / 20k sites; kernel time per traversal (HIP events)."""
+    out = {}
+    base = G.Problem(G.SEQ_DNA, 32, 10000, mean_len=0.1, alpha=0.5, seed=1)
+    cases = [("yule", base.nei, 10000), ("balanced", _topology(32, "balanced"), 10000),
+             ("caterpillar", _topology(32, "caterpillar"), 10000), ("yule", base.nei, 2000),
+             ("yule", base.nei, 20000)]
+    for name, nei, sites in cases:
+        p = G.Problem(G.SEQ_DNA, 32, sites, mean_len=0.1, alpha=0.5, seed=1, topology=nei)
+        row = {"patterns": p.nptn}
+        for prec, fname in ((G.FP64, "fp64"), (G.FP32, "fp32")):
+            r = traversal_rate(p, prec, 50 if quick else 300)
+            row[fname] = {"kernel_us": r["kernel_ms"] * 1e3, "step_us": r["ms_per_step"] * 1e3}
+        out[f"{name}_{sites}"] = row
+        log(f"  {name} {sites} sites ({p.nptn} patterns): fp64 kernel {row['fp64']['kernel_us']:.1f} us, "
+            f"fp32 {row['fp32']['kernel_us']:.1f} us")
+    return out
+
+
 def rell_and_newton(p, o, thr, quick):
     """UFBoot RELL (1000 replicates) and branch-length Newton on the config-2 problem: the
     device against the oracle's restatements on `thr` host cores (RELL: one core)."""
@@ -351,7 +408,7 @@ def main():
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
-    fns = {"1": config1, "2": config2, "3": config3, "4": config4, "5": config5}
+    fns = {"1": config1, "2": config2, "3": config3, "4": config4, "5": config5, "1L": config1_levels}
     res = {}
     for c in args.configs.split(","):
         t0 = time.perf_counter()
